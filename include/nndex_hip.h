@@ -1,0 +1,200 @@
+/*
+ * nndex_hip.h -- C ABI of the MI355X (gfx950) search backend for nndex.
+ *
+ * Two layers are exported by libnndex_hip.so:
+ *
+ *  1. The backend boundary.  It replaces the crate-private `GpuIndex` of
+ *     src/gpu.rs (wgpu + WGSL) one for one: the front-end hands over already
+ *     normalised, zero-padded rows and queries, exactly as src/lib.rs:792-811,
+ *     :353 and :399 do, and gets back owned (index, similarity) lists.  A Rust
+ *     front-end binds these through `extern "C"`.
+ *
+ *  2. The front-end `NNdex` (src/lib.rs:160-599) restated in C++: validation,
+ *     normalisation (on the device), k capping and the error model.  The
+ *     Python package `nndex_amd` binds this layer.
+ *
+ * Plus a device-resident search entry point (inputs and outputs already in
+ * HBM, work enqueued on a caller stream) and small device-memory / stream /
+ * event helpers so callers need no other HIP binding.
+ *
+ * Conventions: every function returns an nndex_status; on failure a message
+ * is kept per thread and returned by nndex_last_error().  Pointers are plain
+ * host pointers unless the name says `device`.  Indices are uint64 (usize in
+ * the reference), similarities float32.
+ */
+#ifndef NNDEX_HIP_H
+#define NNDEX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..7 mirror NNdexError (src/lib.rs:107-145); 10..13 mirror
+ * GpuError (src/gpu.rs:35-53), which the reference wraps in NNdexError::Gpu. */
+typedef enum nndex_status {
+    NNDEX_OK = 0,
+    NNDEX_ERR_SHAPE_MISMATCH = 1,              /* NNdexError::ShapeMismatch */
+    NNDEX_ERR_QUERY_DIM_MISMATCH = 2,          /* NNdexError::QueryDimensionalityMismatch */
+    NNDEX_ERR_INVALID_TOP_K = 3,               /* NNdexError::InvalidTopK */
+    NNDEX_ERR_NON_FINITE_INPUT = 4,            /* NNdexError::NonFiniteInput */
+    NNDEX_ERR_ZERO_NORM = 5,                   /* NNdexError::ZeroNorm */
+    NNDEX_ERR_CPU_BACKEND_UNAVAILABLE = 6,     /* NNdexError::CpuBackendUnavailable */
+    NNDEX_ERR_GPU_BACKEND_UNAVAILABLE = 7,     /* NNdexError::GpuBackendUnavailable */
+    NNDEX_ERR_GPU_ADAPTER_NOT_FOUND = 10,      /* GpuError::AdapterNotFound */
+    NNDEX_ERR_GPU_REQUEST_DEVICE = 11,         /* GpuError::RequestDevice(String) */
+    NNDEX_ERR_GPU_MAP_READ_FAILED = 12,        /* GpuError::MapReadFailed */
+    NNDEX_ERR_GPU_DEVICE_INDEX_OUT_OF_RANGE = 13, /* GpuError::DeviceIndexOutOfRange */
+    /* Boundary-only conditions (no reference counterpart). */
+    NNDEX_ERR_INVALID_ARGUMENT = 20,
+    NNDEX_ERR_OUT_OF_MEMORY = 21,
+    NNDEX_ERR_HIP = 22,
+    NNDEX_ERR_UNSUPPORTED = 23,
+} nndex_status;
+
+/* BackendPreference (src/lib.rs:46-51) / ActiveBackend (src/lib.rs:55-60). */
+typedef enum nndex_backend { NNDEX_BACKEND_CPU = 0, NNDEX_BACKEND_GPU = 1 } nndex_backend;
+
+/* IndexOptions (src/lib.rs:63-89).  gpu_device_index < 0 means None. */
+typedef struct nndex_index_options {
+    int32_t normalized;
+    int32_t approx;
+    int32_t backend; /* nndex_backend */
+    int32_t enable_cache;
+    int64_t gpu_device_index;
+} nndex_index_options;
+
+/* GpuDeviceInfo (src/gpu.rs:974-988); strings are NUL-terminated. */
+typedef struct nndex_gpu_device_info {
+    uint64_t index;
+    char name[128];
+    char driver[64];
+    char driver_info[128];
+    char backend[32];
+    char device_type[32];
+} nndex_gpu_device_info;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nndex_last_error(void);
+/* Library version string. */
+const char *nndex_version(void);
+
+/* ======================================================================== */
+/* 1. Backend boundary (replaces GpuIndex, src/gpu.rs)                      */
+/* ======================================================================== */
+
+typedef struct nndex_gpu_index nndex_gpu_index;
+
+/* GpuIndex::new (src/gpu.rs:138-145).  `storage` is rows x padded_dims f32,
+ * row-major, already normalised and zero-padded; padded_dims % 8 == 0.
+ * device_index < 0 selects the first device (automatic selection).
+ * Errors: DEVICE_INDEX_OUT_OF_RANGE, ADAPTER_NOT_FOUND, OUT_OF_MEMORY. */
+nndex_status nndex_gpu_index_new(const float *storage, size_t rows, size_t padded_dims,
+                                 int32_t approx, int32_t enable_cache, int64_t device_index,
+                                 nndex_gpu_index **out);
+void nndex_gpu_index_free(nndex_gpu_index *index);
+
+/* GpuIndex::search (src/gpu.rs:321-349).  `query` has padded_dims values,
+ * normalised; `k` already capped by the caller (src/lib.rs:347).  Writes
+ * min(k, rows) sorted (similarity desc, index asc) neighbours. */
+nndex_status nndex_gpu_index_search(nndex_gpu_index *index, const float *query, size_t k,
+                                    int32_t approx, uint64_t *out_indices,
+                                    float *out_similarities, size_t *out_len);
+
+/* GpuIndex::search_batch (src/gpu.rs:372-399).  Row q of the output starts at
+ * q*k; out_lens[q] entries are valid. */
+nndex_status nndex_gpu_index_search_batch(nndex_gpu_index *index, const float *queries,
+                                          size_t query_rows, size_t k, int32_t approx,
+                                          uint64_t *out_indices, float *out_similarities,
+                                          size_t *out_lens);
+
+/* GpuIndex::adapter_info (src/gpu.rs:312), reported as in NNdex::gpu_info
+ * (src/lib.rs:302-318, index 0). */
+nndex_status nndex_gpu_index_adapter_info(const nndex_gpu_index *index,
+                                          nndex_gpu_device_info *out);
+
+/* list_gpu_devices (src/gpu.rs:992-1010).  Writes up to `capacity` entries and
+ * the total count; never fails for "no device" (count = 0). */
+nndex_status nndex_list_gpu_devices(nndex_gpu_device_info *out, size_t capacity, size_t *count);
+
+/* Shape of a built index. */
+size_t nndex_gpu_index_rows(const nndex_gpu_index *index);
+size_t nndex_gpu_index_padded_dims(const nndex_gpu_index *index);
+
+/* ======================================================================== */
+/* 2. Device-resident search (no reference counterpart: the reference always */
+/*    round-trips through host buffers).  Inputs and outputs live in device  */
+/*    memory; work is enqueued on `stream` (a hipStream_t, NULL = default)   */
+/*    and the call returns without synchronising.  Calls on one index that   */
+/*    overlap in time must use distinct workspaces.                          */
+/* ======================================================================== */
+
+typedef struct nndex_workspace nndex_workspace;
+
+/* Scratch for up to `max_queries` queries of up to `max_k` neighbours. */
+nndex_status nndex_workspace_new(const nndex_gpu_index *index, size_t max_queries, size_t max_k,
+                                 nndex_workspace **out);
+void nndex_workspace_free(nndex_workspace *ws);
+
+/* Exact FP32 search of `query_rows` normalised padded queries at
+ * `device_queries`; writes query_rows x k (index, similarity) to the device
+ * outputs (k <= rows, k <= the workspace's max_k). */
+nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspace *ws,
+                                           const float *device_queries, size_t query_rows,
+                                           size_t k, uint64_t *device_indices,
+                                           float *device_similarities, void *stream);
+
+/* ======================================================================== */
+/* 3. Front-end NNdex (src/lib.rs) in C++                                   */
+/* ======================================================================== */
+
+typedef struct nndex_index nndex_index;
+
+/* NNdex::new (src/lib.rs:185-278): `matrix` holds `values` floats. */
+nndex_status nndex_new(const float *matrix, size_t values, size_t rows, size_t dims,
+                       const nndex_index_options *options, nndex_index **out);
+void nndex_free(nndex_index *index);
+/* NNdex::backend / rows / dims (src/lib.rs:281-299). */
+int32_t nndex_backend_of(const nndex_index *index);
+size_t nndex_rows(const nndex_index *index);
+size_t nndex_dims(const nndex_index *index);
+/* NNdex::gpu_info (src/lib.rs:302-318); *has_info = 0 for a CPU index. */
+nndex_status nndex_gpu_info(const nndex_index *index, nndex_gpu_device_info *out,
+                            int32_t *has_info);
+/* NNdex::search (src/lib.rs:335-355): query has `query_len` values. */
+nndex_status nndex_search(nndex_index *index, const float *query, size_t query_len, size_t k,
+                          uint64_t *out_indices, float *out_similarities, size_t *out_len);
+/* NNdex::search_batch (src/lib.rs:372-402): queries has `values` floats.
+ * Output rows are strided by min(k, rows). */
+nndex_status nndex_search_batch(nndex_index *index, const float *queries, size_t values,
+                                size_t query_rows, size_t k, uint64_t *out_indices,
+                                float *out_similarities, size_t *out_lens);
+/* The backend handle behind a GPU index (NULL for CPU). */
+nndex_gpu_index *nndex_backend_handle(nndex_index *index);
+
+/* ======================================================================== */
+/* 4. Device helpers                                                        */
+/* ======================================================================== */
+
+nndex_status nndex_device_count(int32_t *count);
+nndex_status nndex_set_device(int32_t device);
+nndex_status nndex_device_malloc(void **ptr, size_t bytes);
+nndex_status nndex_device_free(void *ptr);
+nndex_status nndex_memcpy_h2d(void *device_dst, const void *host_src, size_t bytes);
+nndex_status nndex_memcpy_d2h(void *host_dst, const void *device_src, size_t bytes);
+nndex_status nndex_stream_create(void **stream);
+nndex_status nndex_stream_destroy(void *stream);
+nndex_status nndex_stream_synchronize(void *stream);
+nndex_status nndex_event_create(void **event);
+nndex_status nndex_event_destroy(void *event);
+nndex_status nndex_event_record(void *event, void *stream);
+nndex_status nndex_event_synchronize(void *event);
+nndex_status nndex_event_elapsed_ms(void *start, void *stop, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNDEX_HIP_H */

@@ -1,0 +1,384 @@
+// Exact FP32 search kernels for gfx950: layout transform, row L2
+// normalisation, and the fused score + top-k scan.
+//
+// Reference being replaced: the WGSL dot kernels (src/shaders/dot_lowdim.wgsl,
+// dot_highdim.wgsl), the full-score readback and host heap top-k of
+// src/gpu.rs:511-592 / :640-785, and the host normalisation of
+// src/lib.rs:461-575.
+//
+// Arithmetic follows the reference CPU path exactly: every score is the
+// four-lane f32 FMA chain of simsimd's dot kernel (lane j accumulates elements
+// 4c+j for c = 0, 1, 2, ... in order, then (l0+l1)+(l2+l3)), so scores are
+// bit-identical to the CPU restatement of src/cpu.rs:653-660.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "wave_topk.hpp"
+
+namespace nndex {
+
+// ---------------------------------------------------------------------------
+// Layout transform: row-major [rows][dims] -> tiled float4 layout with zero
+// padding of columns to d4*4 and of rows to a multiple of 64.  Sets bit 0 of
+// *flags if any input value is not finite (ensure_finite, src/lib.rs:445-450).
+// One block handles one 64-row tile; the tile is staged in LDS in column
+// chunks of 64 floats so global reads and writes are both contiguous.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void to_tiled_kernel(const float *__restrict__ src,
+                                                       uint32_t rows, uint32_t dims, uint32_t d4,
+                                                       float4 *__restrict__ dst,
+                                                       uint32_t *__restrict__ flags) {
+    __shared__ float stage[64][64 + 1];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    bool bad = false;
+    for (uint32_t c0 = 0; c0 < d4 * 4; c0 += 64) {
+        // load: 64 rows x 64 columns, consecutive threads take consecutive columns
+        for (uint32_t i = tid; i < 64 * 64; i += 256) {
+            uint32_t r = i / 64, c = i % 64;
+            size_t row = (size_t)tile * 64 + r;
+            uint32_t col = c0 + c;
+            float v = 0.0f;
+            if (row < rows && col < dims) {
+                v = src[row * dims + col];
+                bad |= !__builtin_isfinite(v);
+            }
+            stage[r][c] = v;
+        }
+        __syncthreads();
+        // store: for each float4 chunk, 64 rows contiguous (1 KiB)
+        for (uint32_t i = tid; i < 16 * 64; i += 256) {
+            uint32_t c4l = i / 64, r = i % 64;
+            uint32_t c4 = c0 / 4 + c4l;
+            if (c4 < d4) {
+                float4 v = make_float4(stage[r][c4l * 4 + 0], stage[r][c4l * 4 + 1],
+                                       stage[r][c4l * 4 + 2], stage[r][c4l * 4 + 3]);
+                dst[((size_t)tile * d4 + c4) * 64 + r] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (__ballot(bad) != 0 && __lane_id() == 0) atomicOr(flags, 1u);
+}
+
+// Squared norm with the reference's rules (src/lib.rs:519-553):
+//   dims >= 16: simsimd dot of the row with itself over its `dims` values:
+//               four FMA lanes over full chunks, (l0+l1)+(l2+l3), then the
+//               dims%4 tail elements FMA'd sequentially;
+//   dims <  16: sequential f32 `sq += v*v` without fusion.
+// `get(c4)` returns float4 chunk c4 of the row.
+template <typename Get>
+__device__ inline float norm_sq_ref(Get get, uint32_t dims, bool &nonfinite) {
+    if (dims >= 16) {
+        float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+        uint32_t full = dims / 4;
+        for (uint32_t c = 0; c < full; ++c) {
+            float4 v = get(c);
+            l0 = __builtin_fmaf(v.x, v.x, l0);
+            l1 = __builtin_fmaf(v.y, v.y, l1);
+            l2 = __builtin_fmaf(v.z, v.z, l2);
+            l3 = __builtin_fmaf(v.w, v.w, l3);
+        }
+        float ab = (l0 + l1) + (l2 + l3);
+        uint32_t tail = dims % 4;
+        if (tail) {
+            float4 v = get(full);
+            ab = __builtin_fmaf(v.x, v.x, ab);
+            if (tail > 1) ab = __builtin_fmaf(v.y, v.y, ab);
+            if (tail > 2) ab = __builtin_fmaf(v.z, v.z, ab);
+        }
+        nonfinite = !__builtin_isfinite(ab);
+        return ab;
+    }
+    float sq = 0.f;
+    bool bad = false;
+    for (uint32_t c = 0; c * 4 < dims; ++c) {
+        float4 v = get(c);
+        float e[4] = {v.x, v.y, v.z, v.w};
+        for (uint32_t j = 0; j < 4 && c * 4 + j < dims; ++j) {
+            bad |= !__builtin_isfinite(e[j]);
+            sq = __fadd_rn(sq, __fmul_rn(e[j], e[j]));
+        }
+    }
+    nonfinite = bad;
+    return sq;
+}
+
+// Row normalisation in the tiled layout, one thread per row, in place.
+// Flags: bit 0 non-finite (NNdexError::NonFiniteInput), bit 1 zero norm
+// (NNdexError::ZeroNorm: norm_sq <= f32::EPSILON).
+__global__ __launch_bounds__(256) void normalize_tiled_kernel(float4 *__restrict__ x,
+                                                              uint32_t rows, uint32_t dims,
+                                                              uint32_t d4,
+                                                              uint32_t *__restrict__ flags) {
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    uint32_t f = 0;
+    if (row < rows) {
+        float4 *base = x + (size_t)(row / 64) * d4 * 64 + (row % 64);
+        bool nonfinite = false;
+        float sq = norm_sq_ref([&](uint32_t c) { return base[(size_t)c * 64]; }, dims, nonfinite);
+        if (nonfinite) {
+            f = 1;
+        } else if (sq <= 1.1920929e-7f) {
+            f = 2;
+        } else {
+            float inv = __frcp_rn(__fsqrt_rn(sq));
+            for (uint32_t c = 0; c < d4; ++c) {
+                float4 v = base[(size_t)c * 64];
+                v.x = __fmul_rn(v.x, inv);
+                v.y = __fmul_rn(v.y, inv);
+                v.z = __fmul_rn(v.z, inv);
+                v.w = __fmul_rn(v.w, inv);
+                base[(size_t)c * 64] = v;
+            }
+        }
+    }
+    uint32_t any1 = __ballot(f & 1) != 0, any2 = __ballot(f & 2) != 0;
+    if (__lane_id() == 0 && (any1 || any2)) atomicOr(flags, (any1 ? 1u : 0u) | (any2 ? 2u : 0u));
+}
+
+// Row-major normalisation (queries): src [rows][dims] -> dst [rows][pdims].
+// normalize != 0 applies normalize_rows (src/lib.rs:461-514); otherwise only
+// the finite check and zero padding of pad_rows (src/lib.rs:578-588).
+__global__ __launch_bounds__(256) void normalize_rowmajor_kernel(
+    const float *__restrict__ src, uint32_t rows, uint32_t dims, uint32_t pdims,
+    int normalize, float *__restrict__ dst, uint32_t *__restrict__ flags) {
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    uint32_t f = 0;
+    if (row < rows) {
+        const float *s = src + (size_t)row * dims;
+        float *d = dst + (size_t)row * pdims;
+        auto get = [&](uint32_t c) {
+            float e[4];
+            for (uint32_t j = 0; j < 4; ++j) e[j] = c * 4 + j < dims ? s[c * 4 + j] : 0.f;
+            return make_float4(e[0], e[1], e[2], e[3]);
+        };
+        if (normalize) {
+            bool nonfinite = false;
+            float sq = norm_sq_ref(get, dims, nonfinite);
+            if (nonfinite) {
+                f = 1;
+            } else if (sq <= 1.1920929e-7f) {
+                f = 2;
+            } else {
+                float inv = __frcp_rn(__fsqrt_rn(sq));
+                for (uint32_t i = 0; i < dims; ++i) d[i] = __fmul_rn(s[i], inv);
+            }
+        } else {
+            for (uint32_t i = 0; i < dims; ++i) {
+                float v = s[i];
+                if (!__builtin_isfinite(v)) f = 1;
+                d[i] = v;
+            }
+        }
+        for (uint32_t i = dims; i < pdims; ++i) d[i] = 0.f;
+    }
+    uint32_t any1 = __ballot(f & 1) != 0, any2 = __ballot(f & 2) != 0;
+    if (__lane_id() == 0 && (any1 || any2)) atomicOr(flags, (any1 ? 1u : 0u) | (any2 ? 2u : 0u));
+}
+
+// ---------------------------------------------------------------------------
+// Fused exact scan: scores of QB queries against every row + top-k.
+//
+// Each wave owns a sequence of 64-row tiles; lane r computes the scores of
+// row r of the tile by walking its float4 chunks in order (coalesced 1 KiB
+// loads per chunk; query chunks are wave-uniform and come through the scalar
+// cache).  Scores become ranking keys offered to a per-wave WaveTopK.  At the
+// end the block's four waves are merged, the block writes its sorted top-k to
+// `part`, and the last block to finish (ticket) merges all block lists,
+// writes the outputs and resets the ticket and thresholds for the next launch.
+// ---------------------------------------------------------------------------
+template <int QB, int CW>
+__global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
+    __shared__ uint64_t slots[4][QB][CW];
+    __shared__ uint32_t wave_count[4][QB];
+    __shared__ uint32_t is_last;
+
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t d4 = a.d4;
+    const uint32_t ntiles = (a.rows + 63) / 64;
+    const float4 *__restrict__ X = a.x;
+    const_as<float4> *Q = as_constant(a.q);
+
+    WaveTopK tk[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, a.theta + q);
+
+    for (uint32_t t = blockIdx.x * 4 + wid; t < ntiles; t += gridDim.x * 4) {
+        float acc[QB][4];
+#pragma unroll
+        for (int q = 0; q < QB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.f;
+        const float4 *xt = X + (size_t)t * d4 * 64 + lane;
+#pragma unroll 8
+        for (uint32_t c = 0; c < d4; ++c) {
+            float4 v = load_nt(xt + (size_t)c * 64);
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const uint32_t qq = (uint32_t)q < a.nvalid ? (uint32_t)q : a.nvalid - 1;
+                float4 qv = Q[(size_t)qq * a.q_stride4 + c];
+                acc[q][0] = __builtin_fmaf(v.x, qv.x, acc[q][0]);
+                acc[q][1] = __builtin_fmaf(v.y, qv.y, acc[q][1]);
+                acc[q][2] = __builtin_fmaf(v.z, qv.z, acc[q][2]);
+                acc[q][3] = __builtin_fmaf(v.w, qv.w, acc[q][3]);
+            }
+        }
+        const uint32_t row = t * 64 + lane;
+        const bool valid = row < a.rows;
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            float s = (acc[q][0] + acc[q][1]) + (acc[q][2] + acc[q][3]);
+            tk[q].offer(make_key(s, row), valid);
+        }
+    }
+
+    // Block merge: waves 1..3 hand their lists to wave 0.
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        tk[q].compact();
+        if (lane == 0) wave_count[wid][q] = tk[q].count;
+    }
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            for (uint32_t w = 1; w < 4; ++w) {
+                uint32_t n = wave_count[w][q];
+                for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+                    uint32_t i = i0 + lane;
+                    uint64_t key = i < n ? slots[w][q][i] : 0;
+                    tk[q].offer(key, i < n);
+                }
+            }
+            tk[q].compact();
+            uint64_t *dst = a.part + ((size_t)q * gridDim.x + blockIdx.x) * a.k;
+            for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk[q].count ? slots[0][q][i] : 0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        uint32_t ticket = 0;
+        if (lane == 0)
+            ticket = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+        ticket = __shfl(ticket, 0);
+        if (lane == 0) is_last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+    // Final merge of gridDim.x block lists per query, keys >= final threshold.
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        unsigned long long theta =
+            __hip_atomic_load(a.theta + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tk[q].init(slots[wid][q], CW, a.k, nullptr);
+        for (uint32_t b0 = wid * 64; b0 < gridDim.x; b0 += 256) {
+            const uint32_t b = b0 + lane;
+            const uint64_t *src = a.part + ((size_t)q * gridDim.x + b) * a.k;
+            bool live = b < gridDim.x;
+            for (uint32_t i = 0; i < a.k; ++i) {
+                uint64_t key = live ? src[i] : 0;
+                live = live && key != 0 && key >= theta;
+                if (__ballot(live) == 0) break;
+                tk[q].offer(key, live);
+            }
+        }
+        tk[q].compact();
+        if (lane == 0) wave_count[wid][q] = tk[q].count;
+    }
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            for (uint32_t w = 1; w < 4; ++w) {
+                uint32_t n = wave_count[w][q];
+                for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+                    uint32_t i = i0 + lane;
+                    uint64_t key = i < n ? slots[w][q][i] : 0;
+                    tk[q].offer(key, i < n);
+                }
+            }
+            tk[q].compact();
+            for (uint32_t i = lane; i < a.k && (uint32_t)q < a.nvalid; i += 64) {
+                uint64_t key = i < tk[q].count ? slots[0][q][i] : 0;
+                a.out_idx[(size_t)q * a.out_stride + i] = key ? key_row(key) : 0;
+                a.out_sim[(size_t)q * a.out_stride + i] = key ? key_sim(key) : 0.f;
+            }
+            if (lane == 0)
+                __hip_atomic_store(a.theta + q, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+void launch_to_tiled(const float *src, uint32_t rows, uint32_t dims, uint32_t d4, float4 *dst,
+                     uint32_t *flags, hipStream_t s) {
+    uint32_t tiles = (rows + 63) / 64;
+    hipLaunchKernelGGL(to_tiled_kernel, dim3(tiles), dim3(256), 0, s, src, rows, dims, d4, dst,
+                       flags);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_normalize_tiled(float4 *x, uint32_t rows, uint32_t dims, uint32_t d4,
+                            uint32_t *flags, hipStream_t s) {
+    hipLaunchKernelGGL(normalize_tiled_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, x, rows,
+                       dims, d4, flags);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, uint32_t pdims,
+                               bool normalize, float *dst, uint32_t *flags, hipStream_t s) {
+    hipLaunchKernelGGL(normalize_rowmajor_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, src,
+                       rows, dims, pdims, normalize ? 1 : 0, dst, flags);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+template <int QB, int CW>
+static void launch_scan_t(const ScanArgs &a, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((exact_scan_kernel<QB, CW>), dim3(grid), dim3(256), 0, s, a);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+uint32_t scan_slots_for_k(uint32_t k) {
+    uint32_t cw = 128;
+    while (cw < k + 64) cw <<= 1;
+    return cw;
+}
+
+uint32_t scan_max_qb(uint32_t k) {
+    uint32_t cw = scan_slots_for_k(k);
+    if (cw > 2048) return 0;
+    uint32_t qb = 2048 / cw;  // 4 waves x QB x CW x 8 B <= 64 KiB
+    return qb > 8 ? 8 : qb;
+}
+
+void launch_exact_scan(const ScanArgs &a, uint32_t qb, uint32_t grid, hipStream_t s) {
+    const uint32_t cw = scan_slots_for_k(a.k);
+#define NNDEX_SCAN_CASE(Q, C)                 \
+    if (qb == Q && cw == C) {                 \
+        launch_scan_t<Q, C>(a, grid, s);      \
+        return;                               \
+    }
+    NNDEX_SCAN_CASE(1, 128)
+    NNDEX_SCAN_CASE(1, 256)
+    NNDEX_SCAN_CASE(1, 512)
+    NNDEX_SCAN_CASE(1, 1024)
+    NNDEX_SCAN_CASE(1, 2048)
+    NNDEX_SCAN_CASE(2, 128)
+    NNDEX_SCAN_CASE(2, 256)
+    NNDEX_SCAN_CASE(2, 512)
+    NNDEX_SCAN_CASE(2, 1024)
+    NNDEX_SCAN_CASE(4, 128)
+    NNDEX_SCAN_CASE(4, 256)
+    NNDEX_SCAN_CASE(4, 512)
+    NNDEX_SCAN_CASE(8, 128)
+    NNDEX_SCAN_CASE(8, 256)
+#undef NNDEX_SCAN_CASE
+    throw Error(NNDEX_ERR_UNSUPPORTED, "no exact scan variant for qb=" + std::to_string(qb) +
+                                           " k=" + std::to_string(a.k));
+}
+
+}  // namespace nndex

@@ -1,0 +1,554 @@
+// GpuIndex implementation and the backend C ABI (replaces src/gpu.rs).
+#include "gpu_index.hpp"
+
+#include <cstring>
+#include <string>
+
+namespace nndex {
+
+// ---------------------------------------------------------------------------
+// Per-thread error message
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+void clear_last_error() { g_last_error.clear(); }
+
+// Result caches of src/gpu.rs:21-25: cleared wholesale when full.
+constexpr size_t kSingleCacheCapacity = 128;
+constexpr size_t kBatchCacheCapacity = 16;
+// Rows uploaded per staging chunk during construction.
+constexpr size_t kUploadChunkRows = 1u << 16;
+
+void DeviceBuffer::reserve(size_t n) {
+    if (n <= bytes) return;
+    if (ptr) NNDEX_HIP_CHECK(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    NNDEX_HIP_CHECK(hipMalloc(&ptr, n));
+    bytes = n;
+}
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+void PinnedBuffer::reserve(size_t n) {
+    if (n <= bytes) return;
+    if (ptr) NNDEX_HIP_CHECK(hipHostFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    NNDEX_HIP_CHECK(hipHostMalloc(&ptr, n, hipHostMallocDefault));
+    bytes = n;
+}
+PinnedBuffer::~PinnedBuffer() {
+    if (ptr) (void)hipHostFree(ptr);
+}
+
+Workspace::~Workspace() {
+    if (part) (void)hipFree(part);
+    if (theta) (void)hipFree(theta);
+    if (ticket) (void)hipFree(ticket);
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+void fill_device_info(int device, nndex_gpu_device_info &out) {
+    std::memset(&out, 0, sizeof(out));
+    hipDeviceProp_t p;
+    NNDEX_HIP_CHECK(hipGetDeviceProperties(&p, device));
+    int rt = 0, drv = 0;
+    (void)hipRuntimeGetVersion(&rt);
+    (void)hipDriverGetVersion(&drv);
+    out.index = static_cast<uint64_t>(device);
+    std::snprintf(out.name, sizeof(out.name), "%s", p.name);
+    std::snprintf(out.driver, sizeof(out.driver), "HIP %d", drv);
+    std::snprintf(out.driver_info, sizeof(out.driver_info), "runtime %d, %s, %d CUs", rt,
+                  p.gcnArchName, p.multiProcessorCount);
+    std::snprintf(out.backend, sizeof(out.backend), "Hip");
+    std::snprintf(out.device_type, sizeof(out.device_type), "%s",
+                  p.integrated ? "IntegratedGpu" : "DiscreteGpu");
+}
+
+void throw_flags(uint32_t flags) {
+    if (flags & 1u) throw Error(NNDEX_ERR_NON_FINITE_INPUT, "all input values must be finite");
+    if (flags & 2u) throw Error(NNDEX_ERR_ZERO_NORM, "encountered a zero-norm vector");
+}
+
+// select_adapter (src/gpu.rs:933-970) for HIP devices.
+void GpuIndex::init_device(int64_t device_index) {
+    int count = device_count();
+    if (device_index >= 0) {
+        if (device_index >= count)
+            throw Error(NNDEX_ERR_GPU_DEVICE_INDEX_OUT_OF_RANGE,
+                        "gpu device index " + std::to_string(device_index) +
+                            " is out of range (found " + std::to_string(count) + " adapter(s))");
+        device_ = static_cast<int>(device_index);
+    } else {
+        if (count == 0)
+            throw Error(NNDEX_ERR_GPU_ADAPTER_NOT_FOUND, "no compatible GPU adapter was found");
+        device_ = 0;
+    }
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    fill_device_info(device_, info_);
+    hipDeviceProp_t p;
+    NNDEX_HIP_CHECK(hipGetDeviceProperties(&p, device_));
+    ncu_ = static_cast<uint32_t>(p.multiProcessorCount);
+    NNDEX_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+GpuIndex::~GpuIndex() {
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+    if (x_) (void)hipFree(x_);
+}
+
+void GpuIndex::upload_tiled(const float *host, size_t rows, size_t src_dims, bool check_finite,
+                            bool normalize, size_t logical_dims) {
+    const size_t tiles = padded_rows(rows) / kTileRows;
+    NNDEX_HIP_CHECK(hipMalloc(&x_, tiles * d4_ * kTileRows * sizeof(float4)));
+    DeviceBuffer staging, flags;
+    const size_t chunk = rows < kUploadChunkRows ? rows : kUploadChunkRows;
+    staging.reserve(chunk * src_dims * sizeof(float));
+    flags.reserve(sizeof(uint32_t));
+    NNDEX_HIP_CHECK(hipMemsetAsync(flags.ptr, 0, sizeof(uint32_t), stream_));
+    for (size_t r0 = 0; r0 < rows; r0 += chunk) {
+        size_t n = rows - r0 < chunk ? rows - r0 : chunk;
+        NNDEX_HIP_CHECK(hipMemcpyAsync(staging.ptr, host + r0 * src_dims,
+                                       n * src_dims * sizeof(float), hipMemcpyHostToDevice,
+                                       stream_));
+        launch_to_tiled(static_cast<const float *>(staging.ptr), static_cast<uint32_t>(n),
+                        static_cast<uint32_t>(src_dims), static_cast<uint32_t>(d4_),
+                        x_ + (r0 / kTileRows) * d4_ * kTileRows,
+                        static_cast<uint32_t *>(flags.ptr), stream_);
+    }
+    if (normalize)
+        launch_normalize_tiled(x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(logical_dims),
+                               static_cast<uint32_t>(d4_), static_cast<uint32_t *>(flags.ptr),
+                               stream_);
+    uint32_t h_flags = 0;
+    NNDEX_HIP_CHECK(hipMemcpyAsync(&h_flags, flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   stream_));
+    NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (check_finite || normalize) throw_flags(h_flags);
+}
+
+static void check_rows(size_t rows) {
+    if (rows == 0 || rows >= 0xFFFFFFFFull)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT,
+                    "rows must be in [1, 2^32-1), got " + std::to_string(rows));
+}
+
+std::unique_ptr<GpuIndex> GpuIndex::from_storage(const float *storage, size_t rows,
+                                                 size_t padded_dims, bool approx,
+                                                 bool enable_cache, int64_t device_index) {
+    check_rows(rows);
+    if (padded_dims == 0 || padded_dims % 8 != 0)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "padded_dims must be a positive multiple of 8");
+    std::unique_ptr<GpuIndex> g(new GpuIndex());
+    g->init_device(device_index);
+    g->rows_ = rows;
+    g->pdims_ = padded_dims;
+    g->d4_ = padded_dims / 4;
+    g->approx_ = approx;
+    g->enable_cache_ = enable_cache;
+    g->upload_tiled(storage, rows, padded_dims, false, false, padded_dims);
+    return g;
+}
+
+std::unique_ptr<GpuIndex> GpuIndex::from_raw(const float *matrix, size_t rows, size_t dims,
+                                             bool normalized, bool approx, bool enable_cache,
+                                             int64_t device_index) {
+    check_rows(rows);
+    std::unique_ptr<GpuIndex> g(new GpuIndex());
+    g->init_device(device_index);
+    g->rows_ = rows;
+    g->pdims_ = (dims + 7) / 8 * 8;
+    g->d4_ = g->pdims_ / 4;
+    g->approx_ = approx;
+    g->enable_cache_ = enable_cache;
+    g->upload_tiled(matrix, rows, dims, true, !normalized, dims);
+    return g;
+}
+
+uint32_t GpuIndex::scan_grid() const {
+    const uint32_t tiles = static_cast<uint32_t>(padded_rows(rows_) / kTileRows);
+    const uint32_t want = (tiles + 3) / 4;  // four waves per block, one tile each
+    const uint32_t cap = ncu_ * 4;          // four blocks per CU
+    uint32_t g = want < cap ? want : cap;
+    return g ? g : 1;
+}
+
+std::unique_ptr<Workspace> GpuIndex::make_workspace(size_t max_queries, size_t max_k) const {
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    std::unique_ptr<Workspace> ws(new Workspace());
+    ws->device = device_;
+    ws->max_queries = max_queries;
+    ws->max_k = max_k;
+    ws->grid = scan_grid();
+    // QB * k < 2048 for every fused-scan variant.
+    ws->part_elems = static_cast<size_t>(2048) * ws->grid;
+    NNDEX_HIP_CHECK(hipMalloc(&ws->part, ws->part_elems * sizeof(uint64_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ws->theta, 8 * sizeof(unsigned long long)));
+    NNDEX_HIP_CHECK(hipMalloc(&ws->ticket, sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMemset(ws->theta, 0, 8 * sizeof(unsigned long long)));
+    NNDEX_HIP_CHECK(hipMemset(ws->ticket, 0, sizeof(uint32_t)));
+    return ws;
+}
+
+void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
+                             float *d_sim, hipStream_t s) const {
+    if (k == 0 || k > rows_)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+    if (k > ws.max_k)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k exceeds the workspace's max_k");
+    const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(k));
+    if (qbmax == 0)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
+    for (size_t q0 = 0; q0 < nq; q0 += qbmax) {
+        uint32_t nvalid = static_cast<uint32_t>(nq - q0 < qbmax ? nq - q0 : qbmax);
+        uint32_t qb = 1;
+        while (qb < nvalid) qb <<= 1;
+        ScanArgs a{};
+        a.x = x_;
+        a.rows = static_cast<uint32_t>(rows_);
+        a.d4 = static_cast<uint32_t>(d4_);
+        a.q = reinterpret_cast<const float4 *>(dq + q0 * pdims_);
+        a.q_stride4 = static_cast<uint32_t>(d4_);
+        a.nvalid = nvalid;
+        a.k = static_cast<uint32_t>(k);
+        a.part = ws.part;
+        a.theta = ws.theta;
+        a.ticket = ws.ticket;
+        a.out_idx = d_idx + q0 * k;
+        a.out_sim = d_sim + q0 * k;
+        a.out_stride = static_cast<uint32_t>(k);
+        launch_exact_scan(a, qb, ws.grid, s);
+    }
+}
+
+uint64_t GpuIndex::query_hash(const float *values, size_t n, size_t k, size_t query_rows,
+                              bool approx) const {
+    // FNV-1a over the fields src/gpu.rs:787-797 hashes.
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t bytes) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < bytes; ++i) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    uint64_t f[5] = {rows_, pdims_, k, query_rows, approx ? 1ull : 0ull};
+    mix(f, sizeof(f));
+    mix(values, n * sizeof(float));
+    return h;
+}
+
+std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size_t query_rows,
+                                                      size_t k, size_t raw_dims, bool normalize) {
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    if (!ws_ || ws_->max_k < k) ws_ = make_workspace(8, k < 10 ? 10 : k);
+    const size_t qbytes = query_rows * pdims_ * sizeof(float);
+    const size_t in_bytes = query_rows * (raw_dims ? raw_dims : pdims_) * sizeof(float);
+    const size_t obytes = query_rows * k;
+    d_q_.reserve(qbytes);
+    d_idx_.reserve(obytes * sizeof(uint64_t));
+    d_sim_.reserve(obytes * sizeof(float));
+    h_q_.reserve(in_bytes > sizeof(uint32_t) ? in_bytes : sizeof(uint32_t));
+    h_idx_.reserve(obytes * sizeof(uint64_t));
+    h_sim_.reserve(obytes * sizeof(float));
+    std::memcpy(h_q_.ptr, queries, in_bytes);
+    if (raw_dims) {
+        d_raw_.reserve(in_bytes);
+        d_flags_.reserve(sizeof(uint32_t));
+        NNDEX_HIP_CHECK(hipMemsetAsync(d_flags_.ptr, 0, sizeof(uint32_t), stream_));
+        NNDEX_HIP_CHECK(
+            hipMemcpyAsync(d_raw_.ptr, h_q_.ptr, in_bytes, hipMemcpyHostToDevice, stream_));
+        launch_normalize_rowmajor(static_cast<const float *>(d_raw_.ptr),
+                                  static_cast<uint32_t>(query_rows), static_cast<uint32_t>(raw_dims),
+                                  static_cast<uint32_t>(pdims_), normalize,
+                                  static_cast<float *>(d_q_.ptr),
+                                  static_cast<uint32_t *>(d_flags_.ptr), stream_);
+        uint32_t h_flags = 0;
+        NNDEX_HIP_CHECK(hipMemcpyAsync(&h_flags, d_flags_.ptr, sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, stream_));
+        NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
+        throw_flags(h_flags);
+    } else {
+        NNDEX_HIP_CHECK(
+            hipMemcpyAsync(d_q_.ptr, h_q_.ptr, qbytes, hipMemcpyHostToDevice, stream_));
+    }
+    search_device(*ws_, static_cast<const float *>(d_q_.ptr), query_rows, k,
+                  static_cast<uint64_t *>(d_idx_.ptr), static_cast<float *>(d_sim_.ptr), stream_);
+    NNDEX_HIP_CHECK(hipMemcpyAsync(h_idx_.ptr, d_idx_.ptr, obytes * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, stream_));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(h_sim_.ptr, d_sim_.ptr, obytes * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream_));
+    if (hipStreamSynchronize(stream_) != hipSuccess)
+        throw Error(NNDEX_ERR_GPU_MAP_READ_FAILED, "failed to map GPU buffer for reading");
+    std::vector<std::vector<Neighbor>> out(query_rows);
+    const uint64_t *hi = static_cast<const uint64_t *>(h_idx_.ptr);
+    const float *hs = static_cast<const float *>(h_sim_.ptr);
+    for (size_t q = 0; q < query_rows; ++q) {
+        out[q].resize(k);
+        for (size_t i = 0; i < k; ++i) out[q][i] = {hi[q * k + i], hs[q * k + i]};
+    }
+    return out;
+}
+
+std::vector<Neighbor> GpuIndex::search(const float *query, size_t k, bool approx) {
+    std::lock_guard<std::mutex> lock(mu_);
+    const size_t capped = k < rows_ ? k : rows_;
+    if (enable_cache_) {
+        uint64_t key = query_hash(query, pdims_, k, 1, approx);
+        auto it = cache_single_.find(key);
+        if (it != cache_single_.end()) return it->second;
+        auto res = std::move(run_host(query, 1, capped, 0, false)[0]);
+        if (cache_single_.size() >= kSingleCacheCapacity) cache_single_.clear();
+        cache_single_.emplace(key, res);
+        return res;
+    }
+    return std::move(run_host(query, 1, capped, 0, false)[0]);
+}
+
+std::vector<std::vector<Neighbor>> GpuIndex::search_batch(const float *queries, size_t query_rows,
+                                                          size_t k, bool approx) {
+    std::lock_guard<std::mutex> lock(mu_);
+    const size_t capped = k < rows_ ? k : rows_;
+    if (query_rows == 0) return {};
+    if (enable_cache_) {
+        uint64_t key = query_hash(queries, query_rows * pdims_, k, query_rows, approx);
+        auto it = cache_batch_.find(key);
+        if (it != cache_batch_.end()) return it->second;
+        auto res = run_host(queries, query_rows, capped, 0, false);
+        if (cache_batch_.size() >= kBatchCacheCapacity) cache_batch_.clear();
+        cache_batch_.emplace(key, res);
+        return res;
+    }
+    return run_host(queries, query_rows, capped, 0, false);
+}
+
+std::vector<std::vector<Neighbor>> GpuIndex::search_batch_raw(const float *queries,
+                                                              size_t query_rows, size_t dims,
+                                                              bool normalize, size_t k,
+                                                              bool approx) {
+    std::lock_guard<std::mutex> lock(mu_);
+    const size_t capped = k < rows_ ? k : rows_;
+    if (query_rows == 0) return {};
+    if (enable_cache_) {
+        // Normalisation is deterministic, so keying on the raw values (plus
+        // the mode) hits exactly when a key over the normalised values would.
+        uint64_t key = query_hash(queries, query_rows * dims, k, query_rows, approx) ^
+                       (normalize ? 0x9e3779b97f4a7c15ull : 0ull);
+        auto it = cache_batch_.find(key);
+        if (it != cache_batch_.end()) return it->second;
+        auto res = run_host(queries, query_rows, capped, dims, normalize);
+        if (cache_batch_.size() >= kBatchCacheCapacity) cache_batch_.clear();
+        cache_batch_.emplace(key, res);
+        return res;
+    }
+    return run_host(queries, query_rows, capped, dims, normalize);
+}
+
+}  // namespace nndex
+
+// ===========================================================================
+// C ABI: backend boundary, device-resident search, helpers
+// ===========================================================================
+using namespace nndex;
+
+struct nndex_workspace {
+    std::unique_ptr<Workspace> impl;
+    const GpuIndex *owner;
+};
+
+extern "C" {
+
+const char *nndex_last_error(void) { return g_last_error.c_str(); }
+const char *nndex_version(void) { return "nndex-mi355x 0.1.0 (gfx950)"; }
+
+nndex_status nndex_gpu_index_new(const float *storage, size_t rows, size_t padded_dims,
+                                 int32_t approx, int32_t enable_cache, int64_t device_index,
+                                 nndex_gpu_index **out) {
+    return guard([&] {
+        if (!storage || !out) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        auto h = new nndex_gpu_index();
+        try {
+            h->impl = GpuIndex::from_storage(storage, rows, padded_dims, approx != 0,
+                                             enable_cache != 0, device_index);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void nndex_gpu_index_free(nndex_gpu_index *index) { delete index; }
+// Handles returned by nndex_backend_handle belong to their nndex_index and
+// must not be passed to nndex_gpu_index_free.
+
+size_t nndex_gpu_index_rows(const nndex_gpu_index *index) {
+    return index ? index->impl->rows() : 0;
+}
+size_t nndex_gpu_index_padded_dims(const nndex_gpu_index *index) {
+    return index ? index->impl->padded_dims() : 0;
+}
+
+nndex_status nndex_gpu_index_search(nndex_gpu_index *index, const float *query, size_t k,
+                                    int32_t approx, uint64_t *out_indices,
+                                    float *out_similarities, size_t *out_len) {
+    return guard([&] {
+        if (!index || !query || !out_indices || !out_similarities || !out_len)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        auto res = index->impl->search(query, k, approx != 0);
+        for (size_t i = 0; i < res.size(); ++i) {
+            out_indices[i] = res[i].index;
+            out_similarities[i] = res[i].similarity;
+        }
+        *out_len = res.size();
+    });
+}
+
+nndex_status nndex_gpu_index_search_batch(nndex_gpu_index *index, const float *queries,
+                                          size_t query_rows, size_t k, int32_t approx,
+                                          uint64_t *out_indices, float *out_similarities,
+                                          size_t *out_lens) {
+    return guard([&] {
+        if (!index || (!queries && query_rows) || !out_indices || !out_similarities || !out_lens)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        auto res = index->impl->search_batch(queries, query_rows, k, approx != 0);
+        for (size_t q = 0; q < res.size(); ++q) {
+            for (size_t i = 0; i < res[q].size(); ++i) {
+                out_indices[q * k + i] = res[q][i].index;
+                out_similarities[q * k + i] = res[q][i].similarity;
+            }
+            out_lens[q] = res[q].size();
+        }
+    });
+}
+
+nndex_status nndex_gpu_index_adapter_info(const nndex_gpu_index *index,
+                                          nndex_gpu_device_info *out) {
+    return guard([&] {
+        if (!index || !out) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        *out = index->impl->info();
+        out->index = 0;  // NNdex::gpu_info reports index 0 (src/lib.rs:306)
+    });
+}
+
+nndex_status nndex_list_gpu_devices(nndex_gpu_device_info *out, size_t capacity, size_t *count) {
+    return guard([&] {
+        if (!count) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        int n = device_count();
+        *count = static_cast<size_t>(n);
+        for (int i = 0; i < n && static_cast<size_t>(i) < capacity && out; ++i)
+            fill_device_info(i, out[i]);
+    });
+}
+
+nndex_status nndex_workspace_new(const nndex_gpu_index *index, size_t max_queries, size_t max_k,
+                                 nndex_workspace **out) {
+    return guard([&] {
+        if (!index || !out) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (max_k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        auto w = new nndex_workspace();
+        try {
+            w->impl = index->impl->make_workspace(max_queries, max_k);
+            w->owner = index->impl.get();
+        } catch (...) {
+            delete w;
+            throw;
+        }
+        *out = w;
+    });
+}
+
+void nndex_workspace_free(nndex_workspace *ws) { delete ws; }
+
+nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspace *ws,
+                                           const float *device_queries, size_t query_rows,
+                                           size_t k, uint64_t *device_indices,
+                                           float *device_similarities, void *stream) {
+    return guard([&] {
+        if (!index || !ws || !device_queries || !device_indices || !device_similarities)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (ws->owner != index->impl.get())
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "workspace belongs to another index");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        index->impl->search_device(*ws->impl, device_queries, query_rows, k, device_indices,
+                                   device_similarities, static_cast<hipStream_t>(stream));
+    });
+}
+
+nndex_status nndex_device_count(int32_t *count) {
+    return guard([&] {
+        if (!count) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        *count = device_count();
+    });
+}
+nndex_status nndex_set_device(int32_t device) {
+    return guard([&] { NNDEX_HIP_CHECK(hipSetDevice(device)); });
+}
+nndex_status nndex_device_malloc(void **ptr, size_t bytes) {
+    return guard([&] { NNDEX_HIP_CHECK(hipMalloc(ptr, bytes)); });
+}
+nndex_status nndex_device_free(void *ptr) {
+    return guard([&] { NNDEX_HIP_CHECK(hipFree(ptr)); });
+}
+nndex_status nndex_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    return guard([&] { NNDEX_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+nndex_status nndex_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    return guard([&] { NNDEX_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+nndex_status nndex_stream_create(void **stream) {
+    return guard([&] {
+        hipStream_t s;
+        NNDEX_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+nndex_status nndex_stream_destroy(void *stream) {
+    return guard([&] { NNDEX_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(stream))); });
+}
+nndex_status nndex_stream_synchronize(void *stream) {
+    return guard(
+        [&] { NNDEX_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream))); });
+}
+nndex_status nndex_event_create(void **event) {
+    return guard([&] {
+        hipEvent_t e;
+        NNDEX_HIP_CHECK(hipEventCreate(&e));
+        *event = e;
+    });
+}
+nndex_status nndex_event_destroy(void *event) {
+    return guard([&] { NNDEX_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(event))); });
+}
+nndex_status nndex_event_record(void *event, void *stream) {
+    return guard([&] {
+        NNDEX_HIP_CHECK(
+            hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+    });
+}
+nndex_status nndex_event_synchronize(void *event) {
+    return guard([&] { NNDEX_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(event))); });
+}
+nndex_status nndex_event_elapsed_ms(void *start, void *stop, float *ms) {
+    return guard([&] {
+        NNDEX_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                                            static_cast<hipEvent_t>(stop)));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// GpuIndex: the device-resident index behind the backend C ABI.
+// Replaces `GpuIndex` of src/gpu.rs:93-117.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace nndex {
+
+struct Neighbor {
+    uint64_t index;
+    float similarity;
+};
+
+// Scratch of the fused scan: block lists, thresholds and the ticket.
+struct Workspace {
+    int device = 0;
+    size_t max_queries = 0;
+    size_t max_k = 0;
+    uint32_t grid = 0;
+    uint64_t *part = nullptr;
+    size_t part_elems = 0;
+    unsigned long long *theta = nullptr;  // [8]
+    uint32_t *ticket = nullptr;
+    ~Workspace();
+};
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n);
+    ~DeviceBuffer();
+};
+
+struct PinnedBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n);
+    ~PinnedBuffer();
+};
+
+class GpuIndex {
+  public:
+    // From normalised, padded row-major host storage (src/gpu.rs:138-145).
+    static std::unique_ptr<GpuIndex> from_storage(const float *storage, size_t rows,
+                                                  size_t padded_dims, bool approx,
+                                                  bool enable_cache, int64_t device_index);
+    // From raw host rows: upload, then validate/normalise on the device
+    // (src/lib.rs:239-241 `ensure_finite + pad_rows` or `normalize_rows`).
+    static std::unique_ptr<GpuIndex> from_raw(const float *matrix, size_t rows, size_t dims,
+                                              bool normalized, bool approx, bool enable_cache,
+                                              int64_t device_index);
+    ~GpuIndex();
+
+    size_t rows() const { return rows_; }
+    size_t padded_dims() const { return pdims_; }
+    int device() const { return device_; }
+    const nndex_gpu_device_info &info() const { return info_; }
+    const float4 *tiled() const { return x_; }
+    uint32_t num_cus() const { return ncu_; }
+
+    // GpuIndex::search / search_batch (src/gpu.rs:321-399): host buffers,
+    // normalised padded queries, k already capped.
+    std::vector<Neighbor> search(const float *query, size_t k, bool approx);
+    std::vector<std::vector<Neighbor>> search_batch(const float *queries, size_t query_rows,
+                                                    size_t k, bool approx);
+    // Front-end batch path: raw host queries (query_rows x dims) are checked
+    // and normalised on the device (normalize_rows, src/lib.rs:461-514) or
+    // only checked and padded (ensure_finite + pad_rows) when `normalize` is
+    // false, then searched.
+    std::vector<std::vector<Neighbor>> search_batch_raw(const float *queries, size_t query_rows,
+                                                        size_t dims, bool normalize, size_t k,
+                                                        bool approx);
+
+    // Enqueue an exact search of device-resident queries.
+    void search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
+                       float *d_sim, hipStream_t s) const;
+
+    std::unique_ptr<Workspace> make_workspace(size_t max_queries, size_t max_k) const;
+    uint32_t scan_grid() const;
+
+  private:
+    GpuIndex() = default;
+    void init_device(int64_t device_index);
+    void upload_tiled(const float *host, size_t rows, size_t src_dims, bool check_finite,
+                      bool normalize, size_t logical_dims);
+    uint64_t query_hash(const float *values, size_t n, size_t k, size_t query_rows,
+                        bool approx) const;
+    // raw_dims == 0: queries are normalised and padded already.
+    std::vector<std::vector<Neighbor>> run_host(const float *queries, size_t query_rows,
+                                                size_t k, size_t raw_dims, bool normalize);
+
+    int device_ = 0;
+    uint32_t ncu_ = 0;
+    size_t rows_ = 0, pdims_ = 0, d4_ = 0;
+    bool approx_ = false, enable_cache_ = true;
+    float4 *x_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    nndex_gpu_device_info info_{};
+    std::mutex mu_;
+    std::unique_ptr<Workspace> ws_;
+    DeviceBuffer d_q_, d_raw_, d_idx_, d_sim_, d_flags_;
+    PinnedBuffer h_q_, h_idx_, h_sim_;
+    std::unordered_map<uint64_t, std::vector<Neighbor>> cache_single_;
+    std::unordered_map<uint64_t, std::vector<std::vector<Neighbor>>> cache_batch_;
+};
+
+void fill_device_info(int device, nndex_gpu_device_info &out);
+int device_count();
+
+// Thrown for front-end validation failures raised on the device.
+void throw_flags(uint32_t flags);
+
+}  // namespace nndex
+
+// Opaque C handle of the backend boundary; shared with a front-end index.
+struct nndex_gpu_index {
+    std::shared_ptr<nndex::GpuIndex> impl;
+};

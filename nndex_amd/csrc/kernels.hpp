@@ -1,0 +1,40 @@
+// Kernel launch interfaces shared by the backend translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nndex {
+
+// Arguments of the fused exact scan (exact.hip).
+struct ScanArgs {
+    const float4 *x;              // tiled matrix, rows padded to 64
+    uint32_t rows;                // logical rows
+    uint32_t d4;                  // padded_dims / 4
+    const float4 *q;              // QB queries, row-major, stride q_stride4 float4
+    uint32_t q_stride4;
+    uint32_t nvalid;              // queries actually present (<= QB); later ones repeat the last
+    uint32_t k;                   // neighbours per query (1 <= k <= rows)
+    uint64_t *part;               // [QB][grid][k] block lists
+    unsigned long long *theta;    // [QB] device-wide thresholds (0 between launches)
+    uint32_t *ticket;             // block-completion counter (0 between launches)
+    uint64_t *out_idx;            // [QB][out_stride]
+    float *out_sim;               // [QB][out_stride]
+    uint32_t out_stride;
+};
+
+void launch_to_tiled(const float *src, uint32_t rows, uint32_t dims, uint32_t d4, float4 *dst,
+                     uint32_t *flags, hipStream_t s);
+void launch_normalize_tiled(float4 *x, uint32_t rows, uint32_t dims, uint32_t d4,
+                            uint32_t *flags, hipStream_t s);
+void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, uint32_t pdims,
+                               bool normalize, float *dst, uint32_t *flags, hipStream_t s);
+
+// Slots per wave for a given k (power of two >= k + 64); 0 QB means k is too
+// large for the fused scan.
+uint32_t scan_slots_for_k(uint32_t k);
+uint32_t scan_max_qb(uint32_t k);
+void launch_exact_scan(const ScanArgs &a, uint32_t qb, uint32_t grid, hipStream_t s);
+
+}  // namespace nndex

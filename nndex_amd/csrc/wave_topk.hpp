@@ -1,0 +1,104 @@
+// Wavefront-level bounded top-k over 64-bit ranking keys, kept in LDS.
+//
+// Replaces the host TopKAccumulator loop of the reference GPU path
+// (src/gpu.rs:575-589, :758-776; heap semantics src/topk.rs:44-152) with an
+// on-device selection under the total order (similarity desc, row asc).
+//
+// A wave owns `cap` key slots (cap a power of two, cap >= k + 64).  Candidates
+// above the wave's threshold are appended with a ballot; when the next
+// append could overflow, the wave bitonic-sorts its slots (descending), keeps
+// the first k and raises its threshold to the k-th key.  An optional
+// device-wide threshold (atomic max of every wave's k-th key) is folded in:
+// no key below it can be in the global top-k.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nndex {
+
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+}
+
+struct WaveTopK {
+    uint64_t *buf;    // LDS slots
+    uint32_t cap;     // power of two
+    uint32_t k;       // keys to keep
+    uint32_t count;   // wave-uniform
+    uint64_t thresh;  // wave-uniform: keys must be > thresh
+    unsigned long long *global_thresh;  // device-wide k-th key, or nullptr
+
+    __device__ void init(uint64_t *b, uint32_t c, uint32_t kk, unsigned long long *g) {
+        buf = b;
+        cap = c;
+        k = kk;
+        count = 0;
+        thresh = 0;
+        global_thresh = g;
+    }
+
+    __device__ void sort_desc() {
+        const uint32_t lane = __lane_id();
+        for (uint32_t i = count + lane; i < cap; i += 64) buf[i] = 0;
+        wave_lds_sync();
+        for (uint32_t size = 2; size <= cap; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t p = lane; p < cap / 2; p += 64) {
+                    uint32_t lo = 2 * p - (p & (stride - 1));
+                    uint32_t hi = lo + stride;
+                    bool desc = (lo & size) == 0;
+                    uint64_t a = buf[lo], b = buf[hi];
+                    if ((a < b) == desc) {
+                        buf[lo] = b;
+                        buf[hi] = a;
+                    }
+                }
+                wave_lds_sync();
+            }
+        }
+    }
+
+    // Sort, keep the best k, raise the threshold.
+    __device__ void compact() {
+        sort_desc();
+        if (count > k) count = k;
+        if (k > 0 && count == k) {
+            uint64_t kth = buf[k - 1];
+            if (kth > thresh) thresh = kth;
+            if (global_thresh) {
+                unsigned long long old = 0;
+                if (__lane_id() == 0)
+                    old = __hip_atomic_fetch_max(global_thresh, (unsigned long long)kth,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                old = __shfl(old, 0);
+                uint64_t g = old > kth ? old : kth;
+                if (g - 1 > thresh) thresh = g - 1;
+            }
+        }
+    }
+
+    // Each lane offers at most one key (valid == false offers nothing).
+    __device__ void offer(uint64_t key, bool valid) {
+        bool pass = valid && key > thresh;
+        uint64_t m = __ballot(pass);
+        if (m == 0) return;
+        uint32_t n = __popcll(m);
+        if (count + n > cap) {
+            compact();
+            pass = valid && key > thresh;
+            m = __ballot(pass);
+            if (m == 0) return;
+            n = __popcll(m);
+        }
+        uint32_t pos = count + __popcll(m & __lanemask_lt());
+        if (pass) buf[pos] = key;
+        count += n;
+        wave_lds_sync();
+    }
+};
+
+}  // namespace nndex
