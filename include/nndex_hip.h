@@ -140,11 +140,31 @@ void nndex_workspace_free(nndex_workspace *ws);
 
 /* Exact FP32 search of `query_rows` normalised padded queries at
  * `device_queries`; writes query_rows x k (index, similarity) to the device
- * outputs (k <= rows, k <= the workspace's max_k). */
+ * outputs (k <= rows, k <= the workspace's max_k).  Up to 8 queries run the
+ * fused scan without synchronising; larger batches run the certified MFMA
+ * path and synchronise `stream` once to re-run any uncertified query with
+ * the fused scan, so every result is exact. */
 nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspace *ws,
                                            const float *device_queries, size_t query_rows,
                                            size_t k, uint64_t *device_indices,
                                            float *device_similarities, void *stream);
+
+/* Candidate-scoring precision of the batched path. */
+typedef enum nndex_precision { NNDEX_PRECISION_F32 = 0, NNDEX_PRECISION_BF16 = 1 } nndex_precision;
+
+/* Batched search: MFMA candidate scoring in `precision` (FP32 accumulation),
+ * exact FP32 rerank of the best `overfetch` candidates (0 = default: k + 16
+ * or 1.5k for F32, 4k for BF16), and a per-query certificate written to
+ * device_flags (0 = result proven identical to the exact FP32 search, 1 =
+ * not proven: re-run that query with nndex_gpu_index_search_device).
+ * Fully asynchronous on `stream`.  No reference counterpart (the reference
+ * scores batches in FP32 only and reads every score back, src/gpu.rs:640-785). */
+nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_workspace *ws,
+                                                 const float *device_queries, size_t query_rows,
+                                                 size_t k, int32_t precision, size_t overfetch,
+                                                 uint64_t *device_indices,
+                                                 float *device_similarities,
+                                                 uint8_t *device_flags, void *stream);
 
 /* ======================================================================== */
 /* 3. Front-end NNdex (src/lib.rs) in C++                                   */

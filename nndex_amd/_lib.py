@@ -117,6 +117,8 @@ SIGNATURES = {
     "nndex_workspace_new": (_ST, [_P, _SZ, _SZ, ctypes.POINTER(_P)]),
     "nndex_workspace_free": (None, [_P]),
     "nndex_gpu_index_search_device": (_ST, [_P, _P, _P, _SZ, _SZ, _P, _P, _P]),
+    "nndex_gpu_index_search_batch_device": (_ST, [_P, _P, _P, _SZ, _SZ, _I32, _SZ, _P, _P, _P,
+                                                  _P]),
     "nndex_new": (_ST, [_FP, _SZ, _SZ, _SZ, ctypes.POINTER(IndexOptions), ctypes.POINTER(_P)]),
     "nndex_free": (None, [_P]),
     "nndex_backend_of": (_I32, [_P]),

@@ -1,0 +1,58 @@
+// Launch interfaces of the batched MFMA search (batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nndex {
+
+struct MfmaScanArgs {
+    const uint4 *x;            // tiled matrix in 16-byte units (f32: 4 values, bf16: 8)
+    uint32_t rows;             // logical rows
+    uint32_t rows_padded;      // rows rounded up to 64 (allocated tiles)
+    uint32_t dch;              // 16-byte units per row
+    const uint4 *q;            // queries, row-major, q_stride units per query
+    uint32_t q_stride;
+    uint32_t nq;
+    uint32_t mode;             // 0: append candidates, 1: write scores
+    const unsigned long long *thresh;  // [nq] key thresholds (mode 0)
+    uint64_t *cand;            // [nq][capq] candidate keys (mode 0)
+    uint32_t *cand_count;      // [nq] (mode 0; may exceed capq on overflow)
+    uint32_t capq;
+    float *out;                // [nq][ldo] scores (mode 1)
+    uint32_t ldo;
+};
+
+struct RerankArgs {
+    const float4 *x;           // tiled FP32 master copy
+    uint32_t d4;
+    const float4 *q;           // FP32 queries, row-major, d4 float4 each
+    const uint64_t *cand;
+    const uint32_t *cand_count;
+    uint32_t capq;
+    const unsigned long long *thresh;
+    const float *delta;
+    uint32_t k;                // results per query
+    uint32_t kp;               // rescored candidates per query (k' >= k)
+    uint64_t *out_idx;         // [nq][k]
+    float *out_sim;            // [nq][k]
+    uint8_t *flags;            // [nq] 1 = not certified
+};
+
+void launch_to_bf16(const float4 *x, size_t rows_padded, uint32_t d8, uint4 *y, hipStream_t s);
+void launch_max_norm(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
+                     hipStream_t s);
+void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, bool bf16, uint4 *qlow,
+                         const unsigned long long *xmax_sq, float *delta, hipStream_t s);
+uint32_t mfma_block_rows(bool bf16);
+void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream_t s);
+void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
+                             uint32_t kp, const float *delta, unsigned long long *thresh,
+                             hipStream_t s);
+void launch_rerank(const RerankArgs &a, uint32_t nq, hipStream_t s);
+void launch_gather_rows(const float4 *x, uint32_t d4, uint32_t stride, uint32_t rows_s,
+                        float4 *dst, hipStream_t s);
+void launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+
+}  // namespace nndex

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     const uint32_t d4 = a.d4;
     const uint32_t ntiles = (a.rows + 63) / 64;
     const float4 *__restrict__ X = a.x;
-    const_as<float4> *Q = as_constant(a.q);
+    const_as<f32x4> *Q = as_constant(reinterpret_cast<const f32x4 *>(a.q));
 
     WaveTopK tk[QB];
 #pragma unroll
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
 #pragma unroll
             for (int q = 0; q < QB; ++q) {
                 const uint32_t qq = (uint32_t)q < a.nvalid ? (uint32_t)q : a.nvalid - 1;
-                float4 qv = Q[(size_t)qq * a.q_stride4 + c];
+                const f32x4 qv = Q[(size_t)qq * a.q_stride4 + c];
                 acc[q][0] = __builtin_fmaf(v.x, qv.x, acc[q][0]);
                 acc[q][1] = __builtin_fmaf(v.y, qv.y, acc[q][1]);
                 acc[q][2] = __builtin_fmaf(v.z, qv.z, acc[q][2]);

@@ -44,9 +44,38 @@ PinnedBuffer::~PinnedBuffer() {
 }
 
 Workspace::~Workspace() {
-    if (part) (void)hipFree(part);
-    if (theta) (void)hipFree(theta);
-    if (ticket) (void)hipFree(ticket);
+    for (void *p : {(void *)part, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
+                    (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
+                    (void *)flags})
+        if (p) (void)hipFree(p);
+}
+
+template <typename T>
+static void regrow(T *&p, size_t n) {
+    if (p) NNDEX_HIP_CHECK(hipFree(p));
+    p = nullptr;
+    NNDEX_HIP_CHECK(hipMalloc(&p, n * sizeof(T) + 256));
+}
+
+void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq) {
+    if (nq > b_queries) {
+        regrow(qlow, nq * pdims / 8);
+        regrow(delta, nq);
+        regrow(thresh, nq);
+        regrow(cand_count, nq);
+        regrow(flags, nq);
+        b_queries = nq;
+        b_scores = 0;
+        b_cand = 0;
+    }
+    if (nq * rows_s > b_scores) {
+        regrow(scores, nq * rows_s);
+        b_scores = nq * rows_s;
+    }
+    if (nq * capq > b_cand) {
+        regrow(cand, nq * capq);
+        b_cand = nq * capq;
+    }
 }
 
 int device_count() {
@@ -108,6 +137,12 @@ GpuIndex::~GpuIndex() {
         (void)hipStreamDestroy(stream_);
     }
     if (x_) (void)hipFree(x_);
+    if (xbf16_) (void)hipFree(xbf16_);
+    if (xmax_sq_) (void)hipFree(xmax_sq_);
+    for (auto &smp : samples_) {
+        if (smp.f32) (void)hipFree(smp.f32);
+        if (smp.bf16) (void)hipFree(smp.bf16);
+    }
 }
 
 void GpuIndex::upload_tiled(const float *host, size_t rows, size_t src_dims, bool check_finite,
@@ -133,11 +168,153 @@ void GpuIndex::upload_tiled(const float *host, size_t rows, size_t src_dims, boo
         launch_normalize_tiled(x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(logical_dims),
                                static_cast<uint32_t>(d4_), static_cast<uint32_t *>(flags.ptr),
                                stream_);
+    NNDEX_HIP_CHECK(hipMalloc(&xmax_sq_, sizeof(unsigned long long)));
+    NNDEX_HIP_CHECK(hipMemsetAsync(xmax_sq_, 0, sizeof(unsigned long long), stream_));
+    launch_max_norm(x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(d4_), xmax_sq_,
+                    stream_);
     uint32_t h_flags = 0;
     NNDEX_HIP_CHECK(hipMemcpyAsync(&h_flags, flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                    stream_));
     NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
     if (check_finite || normalize) throw_flags(h_flags);
+}
+
+void GpuIndex::ensure_bf16(hipStream_t s) {
+    std::lock_guard<std::mutex> l(lazy_mu_);
+    if (xbf16_) return;
+    const size_t units = padded_rows(rows_) * (pdims_ / 8);
+    NNDEX_HIP_CHECK(hipMalloc(&xbf16_, units * sizeof(uint4)));
+    launch_to_bf16(x_, padded_rows(rows_), static_cast<uint32_t>(pdims_ / 8), xbf16_, s);
+}
+
+const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s) {
+    std::lock_guard<std::mutex> l(lazy_mu_);
+    RowSample *hit = nullptr;
+    for (auto &smp : samples_)
+        if (smp.stride == stride) hit = &smp;
+    if (!hit) {
+        samples_.push_back(RowSample{});
+        hit = &samples_.back();
+        hit->stride = stride;
+        hit->rows = static_cast<uint32_t>((rows_ + stride - 1) / stride);
+        const size_t prows = padded_rows(hit->rows);
+        NNDEX_HIP_CHECK(hipMalloc(&hit->f32, prows * d4_ * sizeof(float4)));
+        launch_gather_rows(x_, static_cast<uint32_t>(d4_), stride, hit->rows, hit->f32, s);
+    }
+    if (bf16 && !hit->bf16) {
+        const size_t prows = padded_rows(hit->rows);
+        NNDEX_HIP_CHECK(hipMalloc(&hit->bf16, prows * (pdims_ / 8) * sizeof(uint4)));
+        launch_to_bf16(hit->f32, prows, static_cast<uint32_t>(pdims_ / 8), hit->bf16, s);
+    }
+    return *hit;
+}
+
+bool GpuIndex::mfma_supported(Precision p) const {
+    const size_t units = p == Precision::F32 ? pdims_ / 4 : pdims_ / 8;
+    return (units + 3) / 4 <= 32;
+}
+
+size_t GpuIndex::default_overfetch(size_t k, Precision p) {
+    size_t kp = p == Precision::F32 ? k + (k / 2 > 16 ? k / 2 : 16) : 4 * k;
+    return kp > 447 ? 447 : kp;
+}
+
+void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size_t k, Precision p,
+                                 size_t kp, uint64_t *d_idx, float *d_sim, uint8_t *d_flags,
+                                 hipStream_t s) {
+    const bool bf16 = p == Precision::BF16;
+    if (!mfma_supported(p))
+        throw Error(NNDEX_ERR_UNSUPPORTED, "dims too large for the MFMA batch path");
+    if (k == 0 || k > rows_) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+    if (kp == 0) kp = default_overfetch(k, p);
+    if (kp < k) kp = k;
+    if (kp > rows_) kp = rows_;
+    if (kp > 447) throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 447 is not supported");
+    if (bf16) ensure_bf16(s);
+    // sample: about max(256 k', rows/64) rows
+    size_t target = 256 * kp > rows_ / 64 ? 256 * kp : rows_ / 64;
+    uint32_t stride = static_cast<uint32_t>(rows_ / (target ? target : 1));
+    if (stride < 1) stride = 1;
+    if (stride > 64) stride = 64;
+    const RowSample &smp = sample_for(stride, bf16, s);
+    size_t capq = 8 * kp * stride;
+    if (capq < 8192) capq = 8192;
+    const size_t group = 2048;
+    const uint32_t dch = static_cast<uint32_t>(bf16 ? pdims_ / 8 : pdims_ / 4);
+    for (size_t q0 = 0; q0 < nq; q0 += group) {
+        const size_t n = nq - q0 < group ? nq - q0 : group;
+        ws.reserve_batch(n, pdims_, smp.rows, capq);
+        const float *qf = dq + q0 * pdims_;
+        launch_prep_queries(qf, static_cast<uint32_t>(n), static_cast<uint32_t>(pdims_), bf16,
+                            ws.qlow, xmax_sq_, ws.delta, s);
+        const uint4 *qu = bf16 ? ws.qlow : reinterpret_cast<const uint4 *>(qf);
+        // stage 1: sample scores and thresholds
+        MfmaScanArgs a{};
+        a.x = bf16 ? smp.bf16 : reinterpret_cast<const uint4 *>(smp.f32);
+        a.rows = smp.rows;
+        a.rows_padded = static_cast<uint32_t>(padded_rows(smp.rows));
+        a.dch = dch;
+        a.q = qu;
+        a.q_stride = dch;
+        a.nq = static_cast<uint32_t>(n);
+        a.mode = 1;
+        a.out = ws.scores;
+        a.ldo = smp.rows;
+        const uint32_t br = mfma_block_rows(bf16);
+        auto grid_for = [&](uint32_t rp) {
+            uint32_t blocks = (rp + br - 1) / br;
+            uint32_t cap = ncu_ * 2;
+            return blocks < cap ? blocks : cap;
+        };
+        launch_mfma_scan(a, bf16, grid_for(a.rows_padded), s);
+        launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, smp.rows,
+                                static_cast<uint32_t>(kp), ws.delta, ws.thresh, s);
+        // stage 2: filter scan over every row
+        launch_zero_u32(ws.cand_count, n, s);
+        a.x = bf16 ? xbf16_ : reinterpret_cast<const uint4 *>(x_);
+        a.rows = static_cast<uint32_t>(rows_);
+        a.rows_padded = static_cast<uint32_t>(padded_rows(rows_));
+        a.mode = 0;
+        a.thresh = ws.thresh;
+        a.cand = ws.cand;
+        a.cand_count = ws.cand_count;
+        a.capq = static_cast<uint32_t>(capq);
+        launch_mfma_scan(a, bf16, grid_for(a.rows_padded), s);
+        // stages 3-4: exact rerank and certificate
+        RerankArgs r{};
+        r.x = x_;
+        r.d4 = static_cast<uint32_t>(d4_);
+        r.q = reinterpret_cast<const float4 *>(qf);
+        r.cand = ws.cand;
+        r.cand_count = ws.cand_count;
+        r.capq = static_cast<uint32_t>(capq);
+        r.thresh = ws.thresh;
+        r.delta = ws.delta;
+        r.k = static_cast<uint32_t>(k);
+        r.kp = static_cast<uint32_t>(kp);
+        r.out_idx = d_idx + q0 * k;
+        r.out_sim = d_sim + q0 * k;
+        r.flags = d_flags + q0;
+        launch_rerank(r, static_cast<uint32_t>(n), s);
+    }
+}
+
+void GpuIndex::search_exact_any(Workspace &ws, const float *dq, size_t nq, size_t k,
+                                uint64_t *d_idx, float *d_sim, hipStream_t s) {
+    const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(k));
+    if (nq <= 8 || !mfma_supported(Precision::F32) || k > 400 || qbmax == 0) {
+        search_device(ws, dq, nq, k, d_idx, d_sim, s);
+        return;
+    }
+    DeviceBuffer flags;
+    flags.reserve(nq);
+    search_batch_mfma(ws, dq, nq, k, Precision::F32, 0, d_idx, d_sim,
+                      static_cast<uint8_t *>(flags.ptr), s);
+    std::vector<uint8_t> h(nq);
+    NNDEX_HIP_CHECK(hipMemcpyAsync(h.data(), flags.ptr, nq, hipMemcpyDeviceToHost, s));
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+    for (size_t q = 0; q < nq; ++q)
+        if (h[q]) search_device(ws, dq + q * pdims_, 1, k, d_idx + q * k, d_sim + q * k, s);
 }
 
 static void check_rows(size_t rows) {
@@ -285,8 +462,9 @@ std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size
         NNDEX_HIP_CHECK(
             hipMemcpyAsync(d_q_.ptr, h_q_.ptr, qbytes, hipMemcpyHostToDevice, stream_));
     }
-    search_device(*ws_, static_cast<const float *>(d_q_.ptr), query_rows, k,
-                  static_cast<uint64_t *>(d_idx_.ptr), static_cast<float *>(d_sim_.ptr), stream_);
+    search_exact_any(*ws_, static_cast<const float *>(d_q_.ptr), query_rows, k,
+                     static_cast<uint64_t *>(d_idx_.ptr), static_cast<float *>(d_sim_.ptr),
+                     stream_);
     NNDEX_HIP_CHECK(hipMemcpyAsync(h_idx_.ptr, d_idx_.ptr, obytes * sizeof(uint64_t),
                                    hipMemcpyDeviceToHost, stream_));
     NNDEX_HIP_CHECK(hipMemcpyAsync(h_sim_.ptr, d_sim_.ptr, obytes * sizeof(float),
@@ -485,8 +663,31 @@ nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspa
         if (ws->owner != index->impl.get())
             throw Error(NNDEX_ERR_INVALID_ARGUMENT, "workspace belongs to another index");
         if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
-        index->impl->search_device(*ws->impl, device_queries, query_rows, k, device_indices,
-                                   device_similarities, static_cast<hipStream_t>(stream));
+        index->impl->search_exact_any(*ws->impl, device_queries, query_rows, k,
+                                      device_indices, device_similarities,
+                                      static_cast<hipStream_t>(stream));
+    });
+}
+
+nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_workspace *ws,
+                                                 const float *device_queries, size_t query_rows,
+                                                 size_t k, int32_t precision, size_t overfetch,
+                                                 uint64_t *device_indices,
+                                                 float *device_similarities,
+                                                 uint8_t *device_flags, void *stream) {
+    return guard([&] {
+        if (!index || !ws || !device_queries || !device_indices || !device_similarities ||
+            !device_flags)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (ws->owner != index->impl.get())
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "workspace belongs to another index");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        if (precision != NNDEX_PRECISION_F32 && precision != NNDEX_PRECISION_BF16)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "unknown precision");
+        index->impl->search_batch_mfma(*ws->impl, device_queries, query_rows, k,
+                                       static_cast<Precision>(precision), overfetch,
+                                       device_indices, device_similarities, device_flags,
+                                       static_cast<hipStream_t>(stream));
     });
 }
 

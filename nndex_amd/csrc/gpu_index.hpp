@@ -10,6 +10,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -20,7 +21,11 @@ struct Neighbor {
     float similarity;
 };
 
-// Scratch of the fused scan: block lists, thresholds and the ticket.
+// Candidate-scoring precision of the batched path.
+enum class Precision : int { F32 = 0, BF16 = 1 };
+
+// Scratch of the fused scan: block lists, thresholds and the ticket; plus the
+// batched path's buffers, grown on demand.
 struct Workspace {
     int device = 0;
     size_t max_queries = 0;
@@ -30,7 +35,24 @@ struct Workspace {
     size_t part_elems = 0;
     unsigned long long *theta = nullptr;  // [8]
     uint32_t *ticket = nullptr;
+    // batched MFMA path
+    size_t b_queries = 0, b_scores = 0, b_cand = 0;
+    uint4 *qlow = nullptr;
+    float *delta = nullptr;
+    float *scores = nullptr;
+    unsigned long long *thresh = nullptr;
+    uint64_t *cand = nullptr;
+    uint32_t *cand_count = nullptr;
+    uint8_t *flags = nullptr;
+    void reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq);
     ~Workspace();
+};
+
+// Strided row sample used by the batched path's threshold stage.
+struct RowSample {
+    uint32_t stride = 0, rows = 0;
+    float4 *f32 = nullptr;
+    uint4 *bf16 = nullptr;
 };
 
 struct DeviceBuffer {
@@ -84,6 +106,20 @@ class GpuIndex {
     void search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
                        float *d_sim, hipStream_t s) const;
 
+    // Enqueue the batched MFMA search with exact rerank and certificate
+    // (no host synchronisation).  flags[q] = 1 marks a query whose result is
+    // not certified exact; kp = 0 picks the default over-fetch.
+    void search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size_t k, Precision p,
+                           size_t kp, uint64_t *d_idx, float *d_sim, uint8_t *d_flags,
+                           hipStream_t s);
+    // Exact search of any batch: the fused scan for small batches; for larger
+    // ones the F32 MFMA path, then the scan for any uncertified query (this
+    // synchronises `s` once to read the flags).
+    void search_exact_any(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
+                          float *d_sim, hipStream_t s);
+    bool mfma_supported(Precision p) const;
+    static size_t default_overfetch(size_t k, Precision p);
+
     std::unique_ptr<Workspace> make_workspace(size_t max_queries, size_t max_k) const;
     uint32_t scan_grid() const;
 
@@ -103,6 +139,12 @@ class GpuIndex {
     size_t rows_ = 0, pdims_ = 0, d4_ = 0;
     bool approx_ = false, enable_cache_ = true;
     float4 *x_ = nullptr;
+    uint4 *xbf16_ = nullptr;
+    unsigned long long *xmax_sq_ = nullptr;
+    std::mutex lazy_mu_;
+    std::vector<RowSample> samples_;
+    const RowSample &sample_for(uint32_t stride, bool bf16, hipStream_t s);
+    void ensure_bf16(hipStream_t s);
     hipStream_t stream_ = nullptr;
     nndex_gpu_device_info info_{};
     std::mutex mu_;

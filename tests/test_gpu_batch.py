@@ -1,0 +1,124 @@
+"""Batched MFMA path: exact results through the certificate + fallback, and
+the BF16 scoring + FP32 rerank form's recall and certificate rate."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import nndex_amd
+from nndex_amd import _lib, synth
+
+from gpu_util import BackendIndex, DevBuf, DeviceSearch, check, gpu_available
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the HIP path has no CPU fallback")
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def batch_device(b: BackendIndex, ds: DeviceSearch, qs: np.ndarray, k: int, precision: int,
+                 overfetch: int = 0):
+    nq = qs.shape[0]
+    dq = DevBuf.from_array(qs)
+    didx, dsim, dfl = DevBuf(nq * k * 8), DevBuf(nq * k * 4), DevBuf(nq)
+    check(_lib.lib().nndex_gpu_index_search_batch_device(
+        b.h, ds.ws, dq.ptr, nq, k, precision, overfetch, didx.ptr, dsim.ptr, dfl.ptr, ds.stream))
+    check(_lib.lib().nndex_stream_synchronize(ds.stream))
+    return (didx.to_array((nq, k), np.uint64), dsim.to_array((nq, k), np.float32),
+            dfl.to_array((nq,), np.uint8))
+
+
+def exact_device(b: BackendIndex, ds: DeviceSearch, qs: np.ndarray, k: int):
+    nq = qs.shape[0]
+    dq = DevBuf.from_array(qs)
+    didx, dsim = DevBuf(nq * k * 8), DevBuf(nq * k * 4)
+    ds.run(dq, nq, k, didx, dsim)
+    return didx.to_array((nq, k), np.uint64), dsim.to_array((nq, k), np.float32)
+
+
+@pytest.mark.parametrize("rows,dims,nq,k", [
+    (50_000, 128, 40, 10),
+    (20_000, 384, 300, 5),
+    (9_000, 64, 17, 1),
+    (30_000, 96, 64, 100),
+    (3_000, 8, 33, 20),
+])
+def test_exact_batch_through_frontend(orc, rows, dims, nq, k):
+    """Batches above 8 queries take the certified MFMA path; every result must
+    still equal the exact FP32 search bit for bit."""
+    _case("exact_batch", rows=rows, dims=dims, nq=nq, k=k)
+    rng = np.random.default_rng(rows + nq)
+    m = rng.normal(size=(rows, dims)).astype(np.float32)
+    qs = rng.normal(size=(nq, dims)).astype(np.float32)
+    index = nndex_amd.NNdex(m, normalized=False, backend="gpu", enable_cache=False)
+    bidx, bsim = index.search(qs, k=k)
+    om = orc.normalize_rows(m, rows, dims)
+    oq = orc.normalize_rows(qs, nq, dims)
+    for i in range(nq):
+        oidx, osim, _ = orc.cpu_search(om, oq[i], k)
+        assert bidx[i].tolist() == oidx.tolist(), i
+        assert bsim[i].tolist() == osim.tolist(), i
+
+
+def test_f32_batch_device_certifies(orc):
+    _case("f32_certify")
+    base, queries = synth.mixture_workload(rows=200_000, dims=384, n_queries=512)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 512, 100)
+    gi, gs, fl = batch_device(b, ds, queries, 10, 0)
+    ei, es = exact_device(b, ds, queries[:8], 10)
+    assert fl.mean() <= 0.01
+    for i in range(8):
+        if not fl[i]:
+            assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
+    # certified results match the exact scan everywhere we check
+    sample = [i for i in range(0, 512, 37) if not fl[i]]
+    for i in sample:
+        oidx, osim, _ = orc.cpu_search(base, queries[i], 10)
+        assert gi[i].tolist() == oidx.tolist() and gs[i].tolist() == osim.tolist()
+
+
+@pytest.mark.parametrize("dims,k", [(384, 10), (128, 100), (1024, 10)])
+def test_bf16_rerank_recall_and_certificate(orc, dims, k):
+    """BF16 scoring with 4k over-fetch and exact FP32 rerank: certified queries
+    are identical to exact; recall@k over all queries is reported and must be
+    1.0 for the certified ones."""
+    _case("bf16", dims=dims, k=k)
+    rows = 100_000 if dims < 1024 else 40_000
+    base, queries = synth.mixture_workload(rows=rows, dims=dims, n_queries=256)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 256, 4 * k)
+    gi, gs, fl = batch_device(b, ds, queries, k, 1)
+    ei, es = exact_device(b, ds, queries, k)
+    cert = fl == 0
+    print(f"certified {cert.mean():.3f}", flush=True)
+    assert cert.mean() >= 0.5
+    recall = np.mean([len(set(gi[i].tolist()) & set(ei[i].tolist())) / k for i in range(256)])
+    print(f"recall@{k} {recall:.4f}", flush=True)
+    for i in np.nonzero(cert)[0]:
+        assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
+
+
+def test_batch_position_independence_mfma():
+    _case("mfma_position")
+    rng = np.random.default_rng(9)
+    base = rng.normal(size=(60_000, 128)).astype(np.float32)
+    base /= np.linalg.norm(base, axis=1, keepdims=True)
+    qs = rng.normal(size=(200, 128)).astype(np.float32)
+    qs /= np.linalg.norm(qs, axis=1, keepdims=True)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 200, 10)
+    i1, s1 = exact_device(b, ds, qs, 10)
+    perm = rng.permutation(200)
+    i2, s2 = exact_device(b, ds, qs[perm], 10)
+    assert np.array_equal(i1[perm], i2) and np.array_equal(s1[perm], s2)
+    i3, s3 = exact_device(b, ds, qs[:5], 10)  # scan path for the first five
+    assert np.array_equal(i1[:5], i3) and np.array_equal(s1[:5], s3)
