@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--kernel-launches", type=int, default=200)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for profilers: copy calibration + a few searches only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -110,6 +113,12 @@ def main():
     check(L.nndex_gpu_index_adapter_info(h, ctypes.byref(info)))
     stream = ctypes.c_void_p()
     check(L.nndex_stream_create(ctypes.byref(stream)))
+    tune = ""
+    if not args.no_autotune:
+        buf = ctypes.create_string_buffer(4096)
+        check(L.nndex_gpu_index_autotune(h, 20, buf, 4096))
+        tune = buf.value.decode()
+        log("autotune:\n" + tune.strip())
     ws = ctypes.c_void_p()
     check(L.nndex_workspace_new(h, 4096, 128, ctypes.byref(ws)))
 
@@ -122,6 +131,27 @@ def main():
     def step(i, nq=1):
         check(L.nndex_gpu_index_search_device(h, ws, dq.at((i % nbank) * qbytes), nq, k,
                                               didx.ptr, dsim.ptr, stream))
+
+    # ---- copy calibration: a plain streaming copy of the matrix's size ----
+    copy_bytes = rows * dims * 4
+    csrc, cdst = Dev(copy_bytes), Dev(copy_bytes)
+    c0, c1 = event(), event()
+    check(L.nndex_device_copy(cdst.ptr, csrc.ptr, copy_bytes, stream))
+    check(L.nndex_event_record(c0, stream))
+    for _ in range(5):
+        check(L.nndex_device_copy(cdst.ptr, csrc.ptr, copy_bytes, stream))
+    check(L.nndex_event_record(c1, stream))
+    check(L.nndex_stream_synchronize(stream))
+    copy_ms = elapsed(c0, c1) / 5
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"copy kernel: {copy_bytes / 1e9:.3f} GB in {copy_ms * 1e3:.1f} us = "
+        f"{copy_gbs:.0f} GB/s (read+write)")
+    if args.profile:
+        for i in range(20):
+            step(i)
+        check(L.nndex_stream_synchronize(stream))
+        log("profile run done")
+        return
 
     # ---- warmup + timed single-query loop (device-resident) ----
     for i in range(args.warmup):
@@ -193,6 +223,39 @@ def main():
             batch[f"fp32_b{B}"] = {"qps": round(B / dt, 1), "ms_per_batch": round(dt * 1e3, 3)}
             log(f"batch B={B}: {B / dt:.0f} QPS ({dt * 1e3:.2f} ms)")
 
+    # ---- BF16 scoring + exact FP32 rerank at B=4096 vs exact FP32 ----
+    if not args.no_batch:
+        B = 4096
+        ei = Dev(B * k * 8)
+        es = Dev(B * k * 4)
+        check(L.nndex_gpu_index_search_device(h, ws, dq.ptr, B, k, ei.ptr, es.ptr, stream))
+        check(L.nndex_stream_synchronize(stream))
+        exact_i = ei.get((B, k), np.uint64)
+        fl = Dev(B)
+        for prec, name in ((1, "bf16_rerank_b4096"), (0, "f32_mfma_b4096_async")):
+            check(L.nndex_gpu_index_search_batch_device(h, ws, dq.ptr, B, k, prec, 0, didx.ptr,
+                                                        dsim.ptr, fl.ptr, stream))
+            check(L.nndex_stream_synchronize(stream))
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                check(L.nndex_gpu_index_search_batch_device(h, ws, dq.ptr, B, k, prec, 0,
+                                                            didx.ptr, dsim.ptr, fl.ptr, stream))
+            check(L.nndex_stream_synchronize(stream))
+            dt = (time.perf_counter() - t0) / reps
+            gi_b = didx.get((B, k), np.uint64)
+            flags = fl.get((B,), np.uint8)
+            recall = float(np.mean([len(set(gi_b[i].tolist()) & set(exact_i[i].tolist())) / k
+                                    for i in range(B)]))
+            batch[name] = {"qps": round(B / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
+                           "certified": round(1.0 - float(flags.mean()), 4),
+                           f"recall_at_{k}": round(recall, 5)}
+            log(f"{name}: {B / dt:.0f} QPS ({dt * 1e3:.2f} ms), certified "
+                f"{1 - flags.mean():.4f}, recall {recall:.5f}")
+        if "fp32_b4096" in batch:
+            batch["bf16_rerank_speedup_vs_fp32"] = round(
+                batch["bf16_rerank_b4096"]["qps"] / batch["fp32_b4096"]["qps"], 3)
+
     # ---- CPU baseline: the oracle's restatement of the reference path ----
     cpu = None
     if not args.no_cpu:
@@ -242,6 +305,8 @@ def main():
                      "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
         "host_api_qps": round(host_qps, 1),
+        "copy_kernel_gbs": round(copy_gbs, 1),
+        "autotune": tune.strip().splitlines()[-1] if tune else None,
         "batch": batch,
     }
     print(json.dumps(out), flush=True)

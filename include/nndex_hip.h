@@ -166,6 +166,15 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
                                                  float *device_similarities,
                                                  uint8_t *device_flags, void *stream);
 
+/* Autotuner (no reference counterpart): times the fused scan over candidate
+ * launch shapes (64-row tiles per wavefront) on this index and keeps the
+ * fastest; writes a text report.  Workspaces made before the call must be
+ * re-created. */
+nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char *report,
+                                      size_t report_len);
+/* Set the fused scan's tiles per wavefront (0 = default). */
+nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave);
+
 /* ======================================================================== */
 /* 3. Front-end NNdex (src/lib.rs) in C++                                   */
 /* ======================================================================== */
@@ -212,6 +221,9 @@ nndex_status nndex_event_destroy(void *event);
 nndex_status nndex_event_record(void *event, void *stream);
 nndex_status nndex_event_synchronize(void *event);
 nndex_status nndex_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Plain streaming copy kernel (bandwidth calibration). */
+nndex_status nndex_device_copy(void *device_dst, const void *device_src, size_t bytes,
+                               void *stream);
 
 #ifdef __cplusplus
 }

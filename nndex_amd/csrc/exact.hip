@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void normalize_tiled_kernel(float4 *__restrict
         } else if (sq <= 1.1920929e-7f) {
             f = 2;
         } else {
-            float inv = __frcp_rn(__fsqrt_rn(sq));
+            float inv = 1.0f / __builtin_sqrtf(sq);  // both correctly rounded (IEEE), as Rust's sqrt().recip()
             for (uint32_t c = 0; c < d4; ++c) {
                 float4 v = base[(size_t)c * 64];
                 v.x = __fmul_rn(v.x, inv);
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void normalize_rowmajor_kernel(
             } else if (sq <= 1.1920929e-7f) {
                 f = 2;
             } else {
-                float inv = __frcp_rn(__fsqrt_rn(sq));
+                float inv = 1.0f / __builtin_sqrtf(sq);  // both correctly rounded (IEEE), as Rust's sqrt().recip()
                 for (uint32_t i = 0; i < dims; ++i) d[i] = __fmul_rn(s[i], inv);
             }
         } else {
@@ -204,30 +204,83 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
 #pragma unroll
     for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, a.theta + q);
 
-    for (uint32_t t = blockIdx.x * 4 + wid; t < ntiles; t += gridDim.x * 4) {
-        float acc[QB][4];
+    constexpr int U = 8;  // float4 chunks per load group
+    auto load_group = [&](float4(&buf)[U], const float4 *p, uint32_t cbase) {
 #pragma unroll
-        for (int q = 0; q < QB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.f;
-        const float4 *xt = X + (size_t)t * d4 * 64 + lane;
-#pragma unroll 8
-        for (uint32_t c = 0; c < d4; ++c) {
-            float4 v = load_nt(xt + (size_t)c * 64);
+        for (int u = 0; u < U; ++u) buf[u] = load_nt(p + (size_t)(cbase + u) * 64);
+    };
+    auto fma_group = [&](float(&acc)[QB][4], const float4(&buf)[U], uint32_t cbase) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
             for (int q = 0; q < QB; ++q) {
                 const uint32_t qq = (uint32_t)q < a.nvalid ? (uint32_t)q : a.nvalid - 1;
-                const f32x4 qv = Q[(size_t)qq * a.q_stride4 + c];
-                acc[q][0] = __builtin_fmaf(v.x, qv.x, acc[q][0]);
-                acc[q][1] = __builtin_fmaf(v.y, qv.y, acc[q][1]);
-                acc[q][2] = __builtin_fmaf(v.z, qv.z, acc[q][2]);
-                acc[q][3] = __builtin_fmaf(v.w, qv.w, acc[q][3]);
+                const f32x4 qv = Q[(size_t)qq * a.q_stride4 + cbase + u];
+                acc[q][0] = __builtin_fmaf(buf[u].x, qv.x, acc[q][0]);
+                acc[q][1] = __builtin_fmaf(buf[u].y, qv.y, acc[q][1]);
+                acc[q][2] = __builtin_fmaf(buf[u].z, qv.z, acc[q][2]);
+                acc[q][3] = __builtin_fmaf(buf[u].w, qv.w, acc[q][3]);
             }
         }
+    };
+    auto finish_tile = [&](float(&acc)[QB][4], uint32_t t) {
         const uint32_t row = t * 64 + lane;
         const bool valid = row < a.rows;
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
             float s = (acc[q][0] + acc[q][1]) + (acc[q][2] + acc[q][3]);
             tk[q].offer(make_key(s, row), valid);
+        }
+    };
+
+    const uint32_t tstride = gridDim.x * 4;
+    uint32_t t = blockIdx.x * 4 + wid;
+    if (d4 % (2 * U) == 0) {
+        // Two load groups in flight: group g+1 (or the next tile's first
+        // group) is issued before group g is consumed, so the wave keeps
+        // 8-16 KiB of reads outstanding across tile boundaries.
+        float4 A[U], B[U];
+        const float4 *p = X + (size_t)t * d4 * 64 + lane;
+        if (t < ntiles) load_group(A, p, 0);
+        while (t < ntiles) {
+            const uint32_t tn = t + tstride;
+            const float4 *pn = X + (size_t)tn * d4 * 64 + lane;
+            float acc[QB][4];
+#pragma unroll
+            for (int q = 0; q < QB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.f;
+            for (uint32_t c = 0; c < d4; c += 2 * U) {
+                load_group(B, p, c + U);
+                fma_group(acc, A, c);
+                if (c + 2 * U < d4)
+                    load_group(A, p, c + 2 * U);
+                else if (tn < ntiles)
+                    load_group(A, pn, 0);
+                fma_group(acc, B, c + U);
+            }
+            finish_tile(acc, t);
+            t = tn;
+            p = pn;
+        }
+    } else {
+        for (; t < ntiles; t += tstride) {
+            float acc[QB][4];
+#pragma unroll
+            for (int q = 0; q < QB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.f;
+            const float4 *xt = X + (size_t)t * d4 * 64 + lane;
+#pragma unroll 8
+            for (uint32_t c = 0; c < d4; ++c) {
+                float4 v = load_nt(xt + (size_t)c * 64);
+#pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    const uint32_t qq = (uint32_t)q < a.nvalid ? (uint32_t)q : a.nvalid - 1;
+                    const f32x4 qv = Q[(size_t)qq * a.q_stride4 + c];
+                    acc[q][0] = __builtin_fmaf(v.x, qv.x, acc[q][0]);
+                    acc[q][1] = __builtin_fmaf(v.y, qv.y, acc[q][1]);
+                    acc[q][2] = __builtin_fmaf(v.z, qv.z, acc[q][2]);
+                    acc[q][3] = __builtin_fmaf(v.w, qv.w, acc[q][3]);
+                }
+            }
+            finish_tile(acc, t);
         }
     }
 
@@ -339,6 +392,24 @@ void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, u
 template <int QB, int CW>
 static void launch_scan_t(const ScanArgs &a, uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL((exact_scan_kernel<QB, CW>), dim3(grid), dim3(256), 0, s, a);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes % 16)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "copy size must be a multiple of 16 bytes");
+    size_t n = bytes / 16;
+    uint32_t grid = (uint32_t)((n + 255) / 256);
+    if (grid > 256u * 64) grid = 256u * 64;
+    if (grid == 0) return;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 

@@ -95,7 +95,10 @@ void fill_device_info(int device, nndex_gpu_device_info &out) {
     (void)hipRuntimeGetVersion(&rt);
     (void)hipDriverGetVersion(&drv);
     out.index = static_cast<uint64_t>(device);
-    std::snprintf(out.name, sizeof(out.name), "%s", p.name);
+    if (p.name[0])
+        std::snprintf(out.name, sizeof(out.name), "%s", p.name);
+    else  // some drivers report an empty marketing name
+        std::snprintf(out.name, sizeof(out.name), "AMD Instinct GPU (%.40s)", p.gcnArchName);
     std::snprintf(out.driver, sizeof(out.driver), "HIP %d", drv);
     std::snprintf(out.driver_info, sizeof(out.driver_info), "runtime %d, %s, %d CUs", rt,
                   p.gcnArchName, p.multiProcessorCount);
@@ -355,12 +358,72 @@ std::unique_ptr<GpuIndex> GpuIndex::from_raw(const float *matrix, size_t rows, s
     return g;
 }
 
-uint32_t GpuIndex::scan_grid() const {
+uint32_t GpuIndex::scan_tiles_per_wave() const {
+    if (scan_tpw_) return scan_tpw_;
+    // Default: about four tiles per wave, never more waves than 6 per SIMD.
     const uint32_t tiles = static_cast<uint32_t>(padded_rows(rows_) / kTileRows);
-    const uint32_t want = (tiles + 3) / 4;  // four waves per block, one tile each
-    const uint32_t cap = ncu_ * 4;          // four blocks per CU
-    uint32_t g = want < cap ? want : cap;
+    uint32_t m = 4;
+    while ((tiles + 4 * m - 1) / (4 * m) > ncu_ * 6) ++m;
+    return m;
+}
+
+uint32_t GpuIndex::scan_grid() const {
+    // Every wave owns the same number of 64-row tiles (the last wave may own
+    // fewer), so no wave runs a tile longer than the others.
+    const uint32_t tiles = static_cast<uint32_t>(padded_rows(rows_) / kTileRows);
+    const uint32_t m = scan_tiles_per_wave();
+    uint32_t g = (tiles + 4 * m - 1) / (4 * m);
+    const uint32_t cap = ncu_ * 8;
+    if (g > cap) g = cap;
     return g ? g : 1;
+}
+
+std::string GpuIndex::autotune(int reps) {
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    std::lock_guard<std::mutex> lock(mu_);
+    const uint32_t tiles = static_cast<uint32_t>(padded_rows(rows_) / kTileRows);
+    std::vector<float> q(pdims_, 0.f);
+    for (size_t i = 0; i < pdims_; ++i) q[i] = ((i * 2654435761u) % 1000) / 1000.0f - 0.5f;
+    DeviceBuffer dq, di, ds;
+    dq.reserve(pdims_ * sizeof(float));
+    di.reserve(16 * sizeof(uint64_t));
+    ds.reserve(16 * sizeof(float));
+    NNDEX_HIP_CHECK(hipMemcpy(dq.ptr, q.data(), pdims_ * sizeof(float), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    NNDEX_HIP_CHECK(hipEventCreate(&e0));
+    NNDEX_HIP_CHECK(hipEventCreate(&e1));
+    std::string report;
+    float best_ms = 1e30f;
+    uint32_t best_m = 0;
+    for (uint32_t m : {1u, 2u, 3u, 4u, 5u, 6u, 8u, 12u, 16u}) {
+        if ((tiles + 4 * m - 1) / (4 * m) > ncu_ * 8) continue;
+        scan_tpw_ = m;
+        auto ws = make_workspace(1, 10);
+        for (int i = 0; i < 3; ++i)
+            search_device(*ws, static_cast<const float *>(dq.ptr), 1, 10,
+                          static_cast<uint64_t *>(di.ptr), static_cast<float *>(ds.ptr), stream_);
+        NNDEX_HIP_CHECK(hipEventRecord(e0, stream_));
+        for (int i = 0; i < reps; ++i)
+            search_device(*ws, static_cast<const float *>(dq.ptr), 1, 10,
+                          static_cast<uint64_t *>(di.ptr), static_cast<float *>(ds.ptr), stream_);
+        NNDEX_HIP_CHECK(hipEventRecord(e1, stream_));
+        NNDEX_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        NNDEX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        report += "scan tiles_per_wave=" + std::to_string(m) + " grid=" +
+                  std::to_string(scan_grid()) + " us=" + std::to_string(ms * 1e3f) + "\n";
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_m = m;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    scan_tpw_ = best_m;
+    ws_.reset();  // grids changed; the host path re-creates its workspace
+    report += "chosen tiles_per_wave=" + std::to_string(best_m) + "\n";
+    return report;
 }
 
 std::unique_ptr<Workspace> GpuIndex::make_workspace(size_t max_queries, size_t max_k) const {
@@ -407,7 +470,11 @@ void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k
         a.out_idx = d_idx + q0 * k;
         a.out_sim = d_sim + q0 * k;
         a.out_stride = static_cast<uint32_t>(k);
-        launch_exact_scan(a, qb, ws.grid, s);
+        const uint32_t grid = scan_grid();
+        if (static_cast<size_t>(grid) * 2048 > ws.part_elems)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT,
+                        "workspace was created for a smaller launch shape; create a new one");
+        launch_exact_scan(a, qb, grid, s);
     }
 }
 
@@ -688,6 +755,33 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
                                        static_cast<Precision>(precision), overfetch,
                                        device_indices, device_similarities, device_flags,
                                        static_cast<hipStream_t>(stream));
+    });
+}
+
+nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char *report,
+                                     size_t report_len) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        std::string r = index->impl->autotune(reps > 0 ? reps : 20);
+        if (report && report_len) {
+            size_t n = r.size() < report_len - 1 ? r.size() : report_len - 1;
+            std::memcpy(report, r.data(), n);
+            report[n] = 0;
+        }
+    });
+}
+
+nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        index->impl->set_scan_tiles_per_wave(scan_tiles_per_wave);
+    });
+}
+
+nndex_status nndex_device_copy(void *device_dst, const void *device_src, size_t bytes,
+                               void *stream) {
+    return guard([&] {
+        launch_copy(device_dst, device_src, bytes, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -122,6 +123,12 @@ class GpuIndex {
 
     std::unique_ptr<Workspace> make_workspace(size_t max_queries, size_t max_k) const;
     uint32_t scan_grid() const;
+    // Tiles of 64 rows each wave of the fused scan owns (0 = default).
+    void set_scan_tiles_per_wave(uint32_t m) { scan_tpw_ = m; }
+    uint32_t scan_tiles_per_wave() const;
+    // Time the fused scan over candidate launch shapes and keep the fastest;
+    // returns a one-line report per candidate.
+    std::string autotune(int reps);
 
   private:
     GpuIndex() = default;
@@ -136,6 +143,7 @@ class GpuIndex {
 
     int device_ = 0;
     uint32_t ncu_ = 0;
+    uint32_t scan_tpw_ = 0;
     size_t rows_ = 0, pdims_ = 0, d4_ = 0;
     bool approx_ = false, enable_cache_ = true;
     float4 *x_ = nullptr;

@@ -36,5 +36,7 @@ void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, u
 uint32_t scan_slots_for_k(uint32_t k);
 uint32_t scan_max_qb(uint32_t k);
 void launch_exact_scan(const ScanArgs &a, uint32_t qb, uint32_t grid, hipStream_t s);
+// Plain 16-byte streaming copy (bandwidth calibration).
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 }  // namespace nndex
