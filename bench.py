@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--kernel-launches", type=int, default=200)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tiles-per-wave", type=int, default=0,
+                    help="fix the scan launch shape instead of autotuning")
+    ap.add_argument("--scan-flags", type=int, default=1,
+                    help="scan flags with --tiles-per-wave (1 non-temporal, 2 contiguous)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: copy calibration + a few searches only")
     args = ap.parse_args()
@@ -114,7 +118,10 @@ def main():
     stream = ctypes.c_void_p()
     check(L.nndex_stream_create(ctypes.byref(stream)))
     tune = ""
-    if not args.no_autotune:
+    if args.tiles_per_wave:
+        check(L.nndex_gpu_index_set_launch(h, args.tiles_per_wave, args.scan_flags))
+        tune = f"fixed flags={args.scan_flags} tiles_per_wave={args.tiles_per_wave}"
+    elif not args.no_autotune:
         buf = ctypes.create_string_buffer(4096)
         check(L.nndex_gpu_index_autotune(h, 20, buf, 4096))
         tune = buf.value.decode()

@@ -172,8 +172,11 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
  * re-created. */
 nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char *report,
                                       size_t report_len);
-/* Set the fused scan's tiles per wavefront (0 = default). */
-nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave);
+/* Set the fused scan's launch shape: 64-row tiles per wavefront (0 =
+ * default) and flags (bit 0: non-temporal loads, bit 1: contiguous tile runs
+ * per wavefront instead of interleaved tiles). */
+nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave,
+                                        uint32_t scan_flags);
 
 /* ======================================================================== */
 /* 3. Front-end NNdex (src/lib.rs) in C++                                   */

@@ -120,7 +120,7 @@ SIGNATURES = {
     "nndex_gpu_index_search_batch_device": (_ST, [_P, _P, _P, _SZ, _SZ, _I32, _SZ, _P, _P, _P,
                                                   _P]),
     "nndex_gpu_index_autotune": (_ST, [_P, _I32, ctypes.c_char_p, _SZ]),
-    "nndex_gpu_index_set_launch": (_ST, [_P, ctypes.c_uint32]),
+    "nndex_gpu_index_set_launch": (_ST, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     "nndex_device_copy": (_ST, [_P, _P, _SZ, _P]),
     "nndex_new": (_ST, [_FP, _SZ, _SZ, _SZ, ctypes.POINTER(IndexOptions), ctypes.POINTER(_P)]),
     "nndex_free": (None, [_P]),

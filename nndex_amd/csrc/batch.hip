@@ -6,8 +6,8 @@
 // readback, a single-threaded host heap per query) with four device stages:
 //
 //  1. sample threshold: the batch is scored (same MFMA kernel) against a
-//     strided sample of the rows; the k'-th best sample score, lowered by the
-//     per-query error bound delta, is a threshold every true top-k' row passes;
+//     strided sample of the rows; the k'-th best sample score t is the
+//     threshold (at least k' rows reach it);
 //  2. filter scan: MFMA scores of every (row, query) pair; pairs at or above
 //     the threshold are appended to the query's candidate list (no score
 //     matrix is ever written);
@@ -34,25 +34,31 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 // Element traits: one 16-byte unit per lane per MFMA step group.
 // ---------------------------------------------------------------------------
 struct F32E {
-    // four 16x16x4 f32 MFMAs over the unit's four floats (k = 4 each)
-    static __device__ inline v4f mma(uint4 a, uint4 b, v4f c) {
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.x),
-                                                 __builtin_bit_cast(float, b.x), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.y),
-                                                 __builtin_bit_cast(float, b.y), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.z),
-                                                 __builtin_bit_cast(float, b.z), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.w),
-                                                 __builtin_bit_cast(float, b.w), c, 0, 0, 0);
-        return c;
+    // four 16x16x4 f32 MFMAs over the unit's four floats (k = 4 each), each
+    // into its own accumulator so the four chains are independent
+    static constexpr int NACC = 4;
+    static __device__ inline void mma(uint4 a, uint4 b, v4f (&c)[NACC]) {
+        c[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.x),
+                                                    __builtin_bit_cast(float, b.x), c[0], 0, 0, 0);
+        c[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.y),
+                                                    __builtin_bit_cast(float, b.y), c[1], 0, 0, 0);
+        c[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.z),
+                                                    __builtin_bit_cast(float, b.z), c[2], 0, 0, 0);
+        c[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.w),
+                                                    __builtin_bit_cast(float, b.w), c[3], 0, 0, 0);
+    }
+    static __device__ inline float total(const v4f (&c)[NACC], int v) {
+        return (c[0][v] + c[1][v]) + (c[2][v] + c[3][v]);
     }
 };
 struct BF16E {
     // one 16x16x32 bf16 MFMA over the unit's eight bf16 values
-    static __device__ inline v4f mma(uint4 a, uint4 b, v4f c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
-                                                       __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+    static constexpr int NACC = 1;
+    static __device__ inline void mma(uint4 a, uint4 b, v4f (&c)[NACC]) {
+        c[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                       __builtin_bit_cast(v8bf, b), c[0], 0, 0, 0);
     }
+    static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -70,6 +76,7 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float4 *__restrict__
     // unit u of y covers float4 chunks 2c and 2c+1 of the same row
     for (size_t u = (size_t)blockIdx.x * 256 + threadIdx.x; u < units8;
          u += (size_t)gridDim.x * 256) {
+        // (rounding-error norms are accumulated per row by bf16_error_kernel)
         size_t tile = u / ((size_t)d8 * 64);
         size_t rem = u % ((size_t)d8 * 64);
         uint32_t c = rem / 64, r = rem % 64;
@@ -79,6 +86,31 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float4 *__restrict__
         y[u] = make_uint4(pack_bf16x2(a.x, a.y), pack_bf16x2(a.z, a.w), pack_bf16x2(b.x, b.y),
                           pack_bf16x2(b.z, b.w));
     }
+}
+
+// Largest per-row norm of the BF16 rounding error |x~ - x| (f64 bits).
+__global__ __launch_bounds__(256) void bf16_error_kernel(const float4 *__restrict__ x,
+                                                         uint32_t rows, uint32_t d4,
+                                                         unsigned long long *__restrict__ out) {
+    uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    double ss = 0.0;
+    if (row < rows) {
+        const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
+        for (uint32_t c = 0; c < d4; ++c) {
+            float4 v = b[(size_t)c * 64];
+            float e[4] = {v.x, v.y, v.z, v.w};
+            for (int j = 0; j < 4; ++j) {
+                double d = (double)(float)(__bf16)e[j] - (double)e[j];
+                ss += d * d;
+            }
+        }
+    }
+    unsigned long long bits = __builtin_bit_cast(unsigned long long, ss);
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long other = __shfl_xor(bits, o);
+        bits = other > bits ? other : bits;
+    }
+    if (__lane_id() == 0) atomicMax(out, bits);
 }
 
 // Largest row norm of the tiled matrix, in f64 (bounds |x| in delta).
@@ -115,6 +147,7 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
                                                            uint32_t nq, uint32_t pdims,
                                                            int bf16, uint4 *__restrict__ qlow,
                                                            const unsigned long long *xmax_sq,
+                                                           const unsigned long long *xerr_sq,
                                                            float *__restrict__ delta) {
     const uint32_t qi = blockIdx.x;
     if (qi >= nq) return;
@@ -152,38 +185,77 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
         double gamma = n * u / (1.0 - n * u);
         double qn = sqrt(red[0][0]), ql = sqrt(red[1][0]), qd = sqrt(red[2][0]);
         double xl = xm * (1.0 + ue);
-        double d = gamma * xm * qn + gamma * xl * ql + xl * qd + ue * xm * qn;
+        // |sum (x~ - x) q| <= |x~ - x| |q|, with the largest row rounding error
+        // measured when the BF16 copy was made (never above u_e |x|)
+        double xe = bf16 ? sqrt(__builtin_bit_cast(double, *xerr_sq)) : 0.0;
+        if (xe > ue * xm) xe = ue * xm;
+        double d = gamma * xm * qn + gamma * xl * ql + xl * qd + xe * qn;
         delta[qi] = (float)(d * (1.0 + 1e-6)) + 1e-30f;
     }
 }
 
 // ---------------------------------------------------------------------------
 // X-stationary MFMA scan.  NW waves per block, RW rows per wave (multiple of
-// 16), MAXS 16-byte units per lane per row (units per row = 4 * steps; lane
-// group g = lane/16 owns units 4s+g).  Mode 0 appends candidates, mode 1
-// writes scores.
+// 16), MAXS 16-byte units per lane per row (units per row <= 4 * MAXS; lane
+// group g = lane/16 owns units 4s+g), NT 16-query MFMA tiles per query chunk.
+// The next chunk's queries and thresholds are fetched into registers while
+// the current chunk runs on the MFMA units, then written to the other LDS
+// buffer.  Mode 0 appends candidates, mode 1 writes scores.
 // ---------------------------------------------------------------------------
-template <typename E, int NW, int RW, int MAXS>
+template <typename E, int NW, int RW, int MAXS, int NT>
 __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
-    constexpr int MT = RW / 16;  // 16-row MFMA tiles per wave
-    extern __shared__ uint4 qlds[];  // 2 buffers x [4*MAXS units][17 slots]
-    constexpr uint32_t kStride = 17;
+    constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
+    constexpr uint32_t QC = 16 * NT;     // queries per chunk
+    constexpr uint32_t kStride = QC + 1; // LDS slots per unit row (bank spread)
+    constexpr uint32_t NTH = NW * 64;
+    constexpr int STG = (int)((QC * 4 * MAXS + NTH - 1) / NTH);  // staged units per thread
+    extern __shared__ uint4 qlds[];      // 2 buffers x [4*steps units][kStride]
+    const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
-    const uint32_t wid = threadIdx.x >> 6;
+    const uint32_t wid = tid >> 6;
     const uint32_t g = lane >> 4, l16 = lane & 15;
     const uint32_t dch = a.dch;
-    const uint32_t steps = (dch + 3) / 4;
-    const uint32_t units_pad = steps * 4;
-    const uint32_t buf_units = units_pad * kStride;
+    constexpr uint32_t steps = MAXS;  // launcher guarantees 4 * MAXS >= dch
+    const uint32_t buf_units = steps * 4 * kStride;
     const uint32_t block_rows = NW * RW;
     const uint32_t nblk = (a.rows_padded + block_rows - 1) / block_rows;
-    const uint32_t nchunks = (a.nq + 15) / 16;
+    const uint32_t nchunks = (a.nq + QC - 1) / QC;
+    const uint32_t chunk_units = QC * dch;
 
-    // zero the padding units of both buffers once (chunks >= dch)
-    for (uint32_t i = threadIdx.x; i < 2 * buf_units; i += NW * 64) {
+    // zero the padding units of both buffers once (units >= dch)
+    for (uint32_t i = tid; i < 2 * buf_units; i += NTH) {
         uint32_t unit = (i % buf_units) / kStride;
         if (unit >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
     }
+
+    uint4 stg[STG];
+    auto fetch = [&](uint32_t ch) {
+#pragma unroll
+        for (int j = 0; j < STG; ++j) {
+            const uint32_t i = tid + j * NTH;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i < chunk_units) {
+                const uint32_t n = i / dch, c = i % dch, qi = ch * QC + n;
+                if (qi < a.nq) v = a.q[(size_t)qi * a.q_stride + c];
+            }
+            stg[j] = v;
+        }
+    };
+    auto commit = [&](uint32_t buf) {
+        uint4 *dst = qlds + buf * buf_units;
+#pragma unroll
+        for (int j = 0; j < STG; ++j) {
+            const uint32_t i = tid + j * NTH;
+            if (i < chunk_units) dst[(i % dch) * kStride + i / dch] = stg[j];
+        }
+    };
+    auto fetch_thresh = [&](uint32_t ch, unsigned long long (&th)[NT]) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const uint32_t qi = ch * QC + t * 16 + l16;
+            th[t] = (a.mode == 0 && qi < a.nq) ? a.thresh[qi] : ~0ull;
+        }
+    };
 
     for (uint32_t bt = blockIdx.x; bt < nblk; bt += gridDim.x) {
         // ---- stationary X: MT x MAXS units per lane ----
@@ -196,63 +268,75 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
                 const uint32_t c = 4 * s + g;
-                xr[m][s] = (in && s < (int)steps && c < dch) ? base[(size_t)c * 64]
-                                                             : make_uint4(0, 0, 0, 0);
+                xr[m][s] = (in && c < dch) ? base[(size_t)c * 64] : make_uint4(0, 0, 0, 0);
             }
         }
-        // ---- stream query chunks through LDS (double-buffered) ----
-        auto load_chunk = [&](uint32_t ch, uint32_t buf) {
-            uint4 *dst = qlds + buf * buf_units;
-            for (uint32_t i = threadIdx.x; i < 16 * dch; i += NW * 64) {
-                uint32_t n = i / dch, c = i % dch;
-                uint32_t qi = ch * 16 + n;
-                dst[c * kStride + n] =
-                    qi < a.nq ? a.q[(size_t)qi * a.q_stride + c] : make_uint4(0, 0, 0, 0);
-            }
-        };
-        __syncthreads();
-        load_chunk(0, 0);
+        __syncthreads();  // previous block tile done with both buffers
+        fetch(0);
+        commit(0);
+        unsigned long long th_cur[NT], th_next[NT];
+        fetch_thresh(0, th_cur);
         __syncthreads();
         for (uint32_t ch = 0; ch < nchunks; ++ch) {
-            const uint32_t buf = ch & 1;
-            if (ch + 1 < nchunks) load_chunk(ch + 1, buf ^ 1);
-            const uint4 *src = qlds + buf * buf_units;
-            v4f acc[MT];
+            const bool more = ch + 1 < nchunks;
+            if (more) {
+                fetch(ch + 1);
+                fetch_thresh(ch + 1, th_next);
+            }
+            const uint4 *src = qlds + (ch & 1) * buf_units;
+            v4f acc[MT][NT][E::NACC];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) acc[m] = v4f{0.f, 0.f, 0.f, 0.f};
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int j = 0; j < E::NACC; ++j) acc[m][t][j] = v4f{0.f, 0.f, 0.f, 0.f};
+            // one-step software pipeline over LDS reads; the scheduling
+            // barriers keep the compiler from hoisting every step's reads
+            uint4 qc[NT], qn[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) qc[t] = src[g * kStride + t * 16 + l16];
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
-                if (s < (int)steps) {
-                    uint4 qv = src[(4 * s + g) * kStride + l16];
+                if (s + 1 < MAXS) {
 #pragma unroll
-                    for (int m = 0; m < MT; ++m) acc[m] = E::mma(xr[m][s], qv, acc[m]);
+                    for (int t = 0; t < NT; ++t)
+                        qn[t] = src[(4 * (s + 1) + g) * kStride + t * 16 + l16];
                 }
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int m = 0; m < MT; ++m) E::mma(xr[m][s], qc[t], acc[m][t]);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) qc[t] = qn[t];
+                __builtin_amdgcn_sched_barrier(0);
             }
-            // ---- epilogue: lane holds rows 4*g+v of each tile, query l16 ----
-            const uint32_t qi = ch * 16 + l16;
-            if (a.mode == 0) {
-                unsigned long long th = qi < a.nq ? a.thresh[qi] : ~0ull;
+            // ---- epilogue: lane holds rows 4*g+v of each tile, query 16t+l16 ----
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const uint32_t qi = ch * QC + t * 16 + l16;
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
-                        const uint64_t key = make_key(acc[m][v], row);
-                        const bool pass = qi < a.nq && row < a.rows && key >= th;
-                        if (__ballot(pass) != 0 && pass) {
-                            uint32_t pos = atomicAdd(a.cand_count + qi, 1u);
-                            if (pos < a.capq) a.cand[(size_t)qi * a.capq + pos] = key;
+                        if (a.mode == 0) {
+                            const uint64_t key = make_key(E::total(acc[m][t], v), row);
+                            const bool pass = qi < a.nq && row < a.rows && key >= th_cur[t];
+                            if (__ballot(pass) != 0 && pass) {
+                                uint32_t pos = atomicAdd(a.cand_count + qi, 1u);
+                                if (pos < a.capq) a.cand[(size_t)qi * a.capq + pos] = key;
+                            }
+                        } else if (qi < a.nq && row < a.rows) {
+                            a.out[(size_t)qi * a.ldo + row] = E::total(acc[m][t], v);
                         }
                     }
                 }
-            } else {
+            }
+            if (more) {
+                commit((ch + 1) & 1);
 #pragma unroll
-                for (int m = 0; m < MT; ++m)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
-                        if (qi < a.nq && row < a.rows) a.out[(size_t)qi * a.ldo + row] = acc[m][v];
-                    }
+                for (int t = 0; t < NT; ++t) th_cur[t] = th_next[t];
             }
             __syncthreads();
         }
@@ -260,8 +344,8 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Sample threshold: per query, the k'-th best of `n` sample scores, lowered
-// by delta, as a key threshold (keys >= thresh pass).  n < k' -> 0 (all pass).
+// Sample threshold: per query, the k'-th best of `n` sample scores as a key
+// threshold (keys >= thresh pass).  n < k' -> 0 (all pass).
 // ---------------------------------------------------------------------------
 template <int CW>
 __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__restrict__ scores,
@@ -294,17 +378,9 @@ __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__re
             }
         tk.compact();
         if (lane == 0) {
-            float t = key_sim(slots[0][kp - 1]);
-            float lowered = (float)((double)t - (double)delta[q]);
-            // round toward -inf so the float threshold never exceeds t - delta
-            if ((double)lowered > (double)t - (double)delta[q]) {
-                // one ulp toward -inf
-                uint32_t b = __builtin_bit_cast(uint32_t, lowered);
-                if (lowered > 0.f) b -= 1;
-                else if (lowered < 0.f) b += 1;
-                else b = 0x80000001u;
-                lowered = __builtin_bit_cast(float, b);
-            }
+            // At least k' sample rows score >= t; rows under t are bounded by
+            // t + delta in the certificate, so no lowering is needed here.
+            const float lowered = key_sim(slots[0][kp - 1]);
             thresh[q] = (unsigned long long)ordered_bits(lowered) << 32;
         }
     }
@@ -394,6 +470,62 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     }
 }
 
+// Second rerank for queries the first certificate left open: every candidate
+// (all rows whose approximate key passed the threshold) is rescored exactly.
+// The rows left out then all have approx < t_low, so exact < t_low + delta,
+// and the result is exact when t_low + delta < the k-th exact score.
+template <int CW>
+__global__ __launch_bounds__(256) void rerank_full_kernel(RerankArgs a) {
+    __shared__ uint64_t slots[4][CW];
+    __shared__ uint32_t cnt[4];
+    const uint32_t q = blockIdx.x, lane = __lane_id(), wid = threadIdx.x >> 6;
+    if (a.flags[q] == 0) return;
+    const uint32_t total = a.cand_count[q];
+    if (total > a.capq) return;  // overflow: stays flagged for the exact scan
+    const uint64_t *c = a.cand + (size_t)q * a.capq;
+    const float4 *qb = a.q + (size_t)q * a.d4;
+    WaveTopK tk;
+    tk.init(slots[wid], CW, a.k, nullptr);
+    for (uint32_t i0 = wid * 64; i0 < total; i0 += 256) {
+        const uint32_t i = i0 + lane;
+        uint64_t key = 0;
+        if (i < total) {
+            const uint32_t row = key_row(c[i]);
+            const float4 *xb = a.x + (size_t)(row / 64) * a.d4 * 64 + row % 64;
+            float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+            for (uint32_t cc = 0; cc < a.d4; ++cc) {
+                float4 xv = xb[(size_t)cc * 64], qv = qb[cc];
+                l0 = __builtin_fmaf(xv.x, qv.x, l0);
+                l1 = __builtin_fmaf(xv.y, qv.y, l1);
+                l2 = __builtin_fmaf(xv.z, qv.z, l2);
+                l3 = __builtin_fmaf(xv.w, qv.w, l3);
+            }
+            key = make_key((l0 + l1) + (l2 + l3), row);
+        }
+        tk.offer(key, i < total);
+    }
+    tk.compact();
+    if (lane == 0) cnt[wid] = tk.count;
+    __syncthreads();
+    if (wid != 0) return;
+    for (uint32_t w = 1; w < 4; ++w)
+        for (uint32_t i0 = 0; i0 < cnt[w]; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            tk.offer(i < cnt[w] ? slots[w][i] : 0, i < cnt[w]);
+        }
+    tk.compact();
+    if (tk.count < a.k) return;
+    const unsigned long long th = a.thresh[q];
+    const double t_low = th ? (double)from_ordered((uint32_t)(th >> 32)) : -1e300;
+    const bool ok = th == 0 || t_low + (double)a.delta[q] < (double)key_sim(slots[0][a.k - 1]);
+    if (!ok) return;
+    for (uint32_t i = lane; i < a.k; i += 64) {
+        a.out_idx[(size_t)q * a.k + i] = key_row(slots[0][i]);
+        a.out_sim[(size_t)q * a.k + i] = key_sim(slots[0][i]);
+    }
+    if (lane == 0) a.flags[q] = 0;
+}
+
 // Strided row sample of the tiled matrix: sample row i = row i * stride.
 __global__ __launch_bounds__(256) void gather_rows_kernel(const float4 *__restrict__ x, uint32_t d4,
                                                           uint32_t stride, uint32_t rows_s,
@@ -449,6 +581,13 @@ void launch_to_bf16(const float4 *x, size_t rows_padded, uint32_t d8, uint4 *y, 
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
+void launch_bf16_error(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(bf16_error_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, x, rows, d4,
+                       out);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
 void launch_max_norm(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
                      hipStream_t s) {
     hipLaunchKernelGGL(max_norm_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, x, rows, d4,
@@ -457,40 +596,61 @@ void launch_max_norm(const float4 *x, uint32_t rows, uint32_t d4, unsigned long 
 }
 
 void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, bool bf16, uint4 *qlow,
-                         const unsigned long long *xmax_sq, float *delta, hipStream_t s) {
+                         const unsigned long long *xmax_sq, const unsigned long long *xerr_sq,
+                         float *delta, hipStream_t s) {
     hipLaunchKernelGGL(prep_queries_kernel, dim3(nq), dim3(256), 0, s, q, nq, pdims, bf16 ? 1 : 0,
-                       qlow, xmax_sq, delta);
+                       qlow, xmax_sq, xerr_sq, delta);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-template <typename E, int NW, int RW, int MAXS>
+template <typename E, int NW, int RW, int MAXS, int NT>
 static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
-    const uint32_t steps = (a.dch + 3) / 4;
-    const size_t lds = 2ull * steps * 4 * 17 * sizeof(uint4);
-    auto fn = mfma_scan_kernel<E, NW, RW, MAXS>;
+    const size_t lds = 2ull * MAXS * 4 * (16 * NT + 1) * sizeof(uint4);
+    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT>;
     NNDEX_HIP_CHECK(hipFuncSetAttribute((const void *)fn,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(fn, dim3(grid), dim3(NW * 64), lds, s, a);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-uint32_t mfma_block_rows(bool bf16) { return bf16 ? 8 * 32 : 4 * 16; }
+// Step counts with an instantiation; a row of `dch` 16-byte units runs the
+// smallest one with 4 * steps >= dch (extra units are zero).
+static uint32_t mfma_steps(uint32_t dch) {
+    for (uint32_t st : {2u, 4u, 6u, 8u, 12u, 16u, 24u, 32u})
+        if (4 * st >= dch) return st;
+    return 0;
+}
+
+uint32_t mfma_block_rows(bool bf16, uint32_t dch) {
+    return bf16 && mfma_steps(dch) <= 8 ? 8 * 32 : 8 * 16;
+}
 
 void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream_t s) {
-    const uint32_t steps = (a.dch + 3) / 4;
-    if (steps > 32)
+    const uint32_t st = mfma_steps(a.dch);
+    if (st == 0)
         throw Error(NNDEX_ERR_UNSUPPORTED, "padded dims above 512 (f32) / 1024 (bf16) are not "
                                            "supported by the MFMA scan");
-    if (!bf16) {
-        if (steps <= 8) return launch_mfma_t<F32E, 4, 16, 8>(a, grid, s);
-        if (steps <= 16) return launch_mfma_t<F32E, 4, 16, 16>(a, grid, s);
-        if (steps <= 24) return launch_mfma_t<F32E, 4, 16, 24>(a, grid, s);
-        return launch_mfma_t<F32E, 4, 16, 32>(a, grid, s);
+#define NNDEX_MFMA_CASE(STEPS, F32_NT, BF_RW, BF_NT)                                   \
+    case STEPS:                                                                        \
+        if (bf16)                                                                      \
+            return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT>(a, grid, s);           \
+        return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT>(a, grid, s);
+    switch (st) {
+        // (steps, F32 query tiles, BF16 rows per wave, BF16 query tiles):
+        // the largest tiles that fit 256 VGPRs without spills
+        NNDEX_MFMA_CASE(2, 4, 32, 4)
+        NNDEX_MFMA_CASE(4, 4, 32, 4)
+        NNDEX_MFMA_CASE(6, 2, 32, 2)
+        NNDEX_MFMA_CASE(8, 2, 32, 2)
+        NNDEX_MFMA_CASE(12, 2, 16, 4)
+        NNDEX_MFMA_CASE(16, 2, 16, 2)
+        NNDEX_MFMA_CASE(24, 1, 16, 1)
+        NNDEX_MFMA_CASE(32, 1, 16, 1)
+    default:
+        break;
     }
-    if (steps <= 8) return launch_mfma_t<BF16E, 8, 32, 8>(a, grid, s);
-    if (steps <= 16) return launch_mfma_t<BF16E, 8, 32, 16>(a, grid, s);
-    if (steps <= 24) return launch_mfma_t<BF16E, 8, 16, 24>(a, grid, s);
-    return launch_mfma_t<BF16E, 8, 16, 32>(a, grid, s);
+#undef NNDEX_MFMA_CASE
+    throw Error(NNDEX_ERR_UNSUPPORTED, "no MFMA scan variant");
 }
 
 uint32_t cw_for(uint32_t k) {
@@ -517,6 +677,23 @@ void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint3
         break;
     default:
         throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 448 is not supported");
+    }
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_rerank_full(const RerankArgs &a, uint32_t nq, hipStream_t s) {
+    switch (cw_for(a.k)) {
+    case 128:
+        hipLaunchKernelGGL(rerank_full_kernel<128>, dim3(nq), dim3(256), 0, s, a);
+        break;
+    case 256:
+        hipLaunchKernelGGL(rerank_full_kernel<256>, dim3(nq), dim3(256), 0, s, a);
+        break;
+    case 512:
+        hipLaunchKernelGGL(rerank_full_kernel<512>, dim3(nq), dim3(256), 0, s, a);
+        break;
+    default:
+        return;  // k above 448: queries stay flagged for the exact scan
     }
     NNDEX_HIP_CHECK(hipGetLastError());
 }

@@ -44,13 +44,17 @@ void launch_to_bf16(const float4 *x, size_t rows_padded, uint32_t d8, uint4 *y, 
 void launch_max_norm(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
                      hipStream_t s);
 void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, bool bf16, uint4 *qlow,
-                         const unsigned long long *xmax_sq, float *delta, hipStream_t s);
-uint32_t mfma_block_rows(bool bf16);
+                         const unsigned long long *xmax_sq, const unsigned long long *xerr_sq,
+                         float *delta, hipStream_t s);
+void launch_bf16_error(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
+                       hipStream_t s);
+uint32_t mfma_block_rows(bool bf16, uint32_t dch);
 void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream_t s);
 void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
                              uint32_t kp, const float *delta, unsigned long long *thresh,
                              hipStream_t s);
 void launch_rerank(const RerankArgs &a, uint32_t nq, hipStream_t s);
+void launch_rerank_full(const RerankArgs &a, uint32_t nq, hipStream_t s);
 void launch_gather_rows(const float4 *x, uint32_t d4, uint32_t stride, uint32_t rows_s,
                         float4 *dst, hipStream_t s);
 void launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);

@@ -205,9 +205,15 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, a.theta + q);
 
     constexpr int U = 8;  // float4 chunks per load group
+    const bool nt = a.flags & kScanNonTemporal;
     auto load_group = [&](float4(&buf)[U], const float4 *p, uint32_t cbase) {
+        if (nt) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) buf[u] = load_nt(p + (size_t)(cbase + u) * 64);
+            for (int u = 0; u < U; ++u) buf[u] = load_nt(p + (size_t)(cbase + u) * 64);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) buf[u] = p[(size_t)(cbase + u) * 64];
+        }
     };
     auto fma_group = [&](float(&acc)[QB][4], const float4(&buf)[U], uint32_t cbase) {
 #pragma unroll
@@ -233,16 +239,29 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
         }
     };
 
-    const uint32_t tstride = gridDim.x * 4;
-    uint32_t t = blockIdx.x * 4 + wid;
+    // Tile order: interleaved (wave w takes w, w + W, ...) or contiguous
+    // (wave w takes a run of ceil(ntiles / W) consecutive tiles).
+    const uint32_t nwaves = gridDim.x * 4;
+    const uint32_t wg = blockIdx.x * 4 + wid;
+    uint32_t t, tend, tstride;
+    if (a.flags & kScanContiguous) {
+        const uint32_t run = (ntiles + nwaves - 1) / nwaves;
+        t = wg * run;
+        tend = t + run < ntiles ? t + run : ntiles;
+        tstride = 1;
+    } else {
+        t = wg;
+        tend = ntiles;
+        tstride = nwaves;
+    }
     if (d4 % (2 * U) == 0) {
         // Two load groups in flight: group g+1 (or the next tile's first
         // group) is issued before group g is consumed, so the wave keeps
         // 8-16 KiB of reads outstanding across tile boundaries.
         float4 A[U], B[U];
         const float4 *p = X + (size_t)t * d4 * 64 + lane;
-        if (t < ntiles) load_group(A, p, 0);
-        while (t < ntiles) {
+        if (t < tend) load_group(A, p, 0);
+        while (t < tend) {
             const uint32_t tn = t + tstride;
             const float4 *pn = X + (size_t)tn * d4 * 64 + lane;
             float acc[QB][4];
@@ -253,7 +272,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
                 fma_group(acc, A, c);
                 if (c + 2 * U < d4)
                     load_group(A, p, c + 2 * U);
-                else if (tn < ntiles)
+                else if (tn < tend)
                     load_group(A, pn, 0);
                 fma_group(acc, B, c + U);
             }
@@ -262,7 +281,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
             p = pn;
         }
     } else {
-        for (; t < ntiles; t += tstride) {
+        for (; t < tend; t += tstride) {
             float acc[QB][4];
 #pragma unroll
             for (int q = 0; q < QB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.f;
@@ -303,8 +322,18 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
                 }
             }
             tk[q].compact();
+            // Publish only the keys at or above the device-wide threshold as
+            // it stands now (it only grows), with their count.
+            const unsigned long long th =
+                __hip_atomic_load(a.theta + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t live = 0;
+            for (uint32_t i0 = 0; i0 < tk[q].count; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                live += __popcll(__ballot(i < tk[q].count && slots[0][q][i] >= th));
+            }
             uint64_t *dst = a.part + ((size_t)q * gridDim.x + blockIdx.x) * a.k;
-            for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk[q].count ? slots[0][q][i] : 0;
+            for (uint32_t i = lane; i < live; i += 64) dst[i] = slots[0][q][i];
+            if (lane == 0) a.part_count[(size_t)q * gridDim.x + blockIdx.x] = live;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t ticket = 0;
@@ -327,12 +356,15 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
         for (uint32_t b0 = wid * 64; b0 < gridDim.x; b0 += 256) {
             const uint32_t b = b0 + lane;
             const uint64_t *src = a.part + ((size_t)q * gridDim.x + b) * a.k;
-            bool live = b < gridDim.x;
-            for (uint32_t i = 0; i < a.k; ++i) {
-                uint64_t key = live ? src[i] : 0;
-                live = live && key != 0 && key >= theta;
-                if (__ballot(live) == 0) break;
-                tk[q].offer(key, live);
+            const uint32_t cnt = b < gridDim.x ? a.part_count[(size_t)q * gridDim.x + b] : 0;
+            uint32_t maxc = cnt;
+            for (int off = 32; off > 0; off >>= 1) {
+                uint32_t o = __shfl_xor(maxc, off);
+                maxc = o > maxc ? o : maxc;
+            }
+            for (uint32_t i = 0; i < maxc; ++i) {
+                const uint64_t key = i < cnt ? src[i] : 0;
+                tk[q].offer(key, i < cnt && key >= theta);
             }
         }
         tk[q].compact();

@@ -44,7 +44,7 @@ PinnedBuffer::~PinnedBuffer() {
 }
 
 Workspace::~Workspace() {
-    for (void *p : {(void *)part, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
+    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
                     (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
                     (void *)flags})
         if (p) (void)hipFree(p);
@@ -142,6 +142,7 @@ GpuIndex::~GpuIndex() {
     if (x_) (void)hipFree(x_);
     if (xbf16_) (void)hipFree(xbf16_);
     if (xmax_sq_) (void)hipFree(xmax_sq_);
+    if (xerr_sq_) (void)hipFree(xerr_sq_);
     for (auto &smp : samples_) {
         if (smp.f32) (void)hipFree(smp.f32);
         if (smp.bf16) (void)hipFree(smp.bf16);
@@ -188,6 +189,10 @@ void GpuIndex::ensure_bf16(hipStream_t s) {
     const size_t units = padded_rows(rows_) * (pdims_ / 8);
     NNDEX_HIP_CHECK(hipMalloc(&xbf16_, units * sizeof(uint4)));
     launch_to_bf16(x_, padded_rows(rows_), static_cast<uint32_t>(pdims_ / 8), xbf16_, s);
+    NNDEX_HIP_CHECK(hipMalloc(&xerr_sq_, 2 * sizeof(unsigned long long)));
+    NNDEX_HIP_CHECK(hipMemsetAsync(xerr_sq_, 0, 2 * sizeof(unsigned long long), s));
+    launch_bf16_error(x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(d4_), xerr_sq_ + 1,
+                      s);
 }
 
 const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s) {
@@ -218,7 +223,8 @@ bool GpuIndex::mfma_supported(Precision p) const {
 }
 
 size_t GpuIndex::default_overfetch(size_t k, Precision p) {
-    size_t kp = p == Precision::F32 ? k + (k / 2 > 16 ? k / 2 : 16) : 4 * k;
+    size_t kp = p == Precision::F32 ? k + (k / 2 > 16 ? k / 2 : 16)
+                                    : (4 * k > k + 64 ? 4 * k : k + 64);
     return kp > 447 ? 447 : kp;
 }
 
@@ -234,8 +240,8 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
     if (kp > rows_) kp = rows_;
     if (kp > 447) throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 447 is not supported");
     if (bf16) ensure_bf16(s);
-    // sample: about max(256 k', rows/64) rows
-    size_t target = 256 * kp > rows_ / 64 ? 256 * kp : rows_ / 64;
+    // sample: about max(256 k', rows/16) rows -> about 16 k' candidates
+    size_t target = 256 * kp > rows_ / 16 ? 256 * kp : rows_ / 16;
     uint32_t stride = static_cast<uint32_t>(rows_ / (target ? target : 1));
     if (stride < 1) stride = 1;
     if (stride > 64) stride = 64;
@@ -249,7 +255,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         ws.reserve_batch(n, pdims_, smp.rows, capq);
         const float *qf = dq + q0 * pdims_;
         launch_prep_queries(qf, static_cast<uint32_t>(n), static_cast<uint32_t>(pdims_), bf16,
-                            ws.qlow, xmax_sq_, ws.delta, s);
+                            ws.qlow, xmax_sq_, bf16 ? xerr_sq_ + 1 : xmax_sq_, ws.delta, s);
         const uint4 *qu = bf16 ? ws.qlow : reinterpret_cast<const uint4 *>(qf);
         // stage 1: sample scores and thresholds
         MfmaScanArgs a{};
@@ -263,10 +269,10 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.mode = 1;
         a.out = ws.scores;
         a.ldo = smp.rows;
-        const uint32_t br = mfma_block_rows(bf16);
+        const uint32_t br = mfma_block_rows(bf16, dch);
         auto grid_for = [&](uint32_t rp) {
             uint32_t blocks = (rp + br - 1) / br;
-            uint32_t cap = ncu_ * 2;
+            uint32_t cap = ncu_;  // one 8-wave block per CU (register-bound)
             return blocks < cap ? blocks : cap;
         };
         launch_mfma_scan(a, bf16, grid_for(a.rows_padded), s);
@@ -299,6 +305,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         r.out_sim = d_sim + q0 * k;
         r.flags = d_flags + q0;
         launch_rerank(r, static_cast<uint32_t>(n), s);
+        launch_rerank_full(r, static_cast<uint32_t>(n), s);
     }
 }
 
@@ -394,35 +401,44 @@ std::string GpuIndex::autotune(int reps) {
     NNDEX_HIP_CHECK(hipEventCreate(&e1));
     std::string report;
     float best_ms = 1e30f;
-    uint32_t best_m = 0;
-    for (uint32_t m : {1u, 2u, 3u, 4u, 5u, 6u, 8u, 12u, 16u}) {
-        if ((tiles + 4 * m - 1) / (4 * m) > ncu_ * 8) continue;
-        scan_tpw_ = m;
-        auto ws = make_workspace(1, 10);
-        for (int i = 0; i < 3; ++i)
-            search_device(*ws, static_cast<const float *>(dq.ptr), 1, 10,
-                          static_cast<uint64_t *>(di.ptr), static_cast<float *>(ds.ptr), stream_);
-        NNDEX_HIP_CHECK(hipEventRecord(e0, stream_));
-        for (int i = 0; i < reps; ++i)
-            search_device(*ws, static_cast<const float *>(dq.ptr), 1, 10,
-                          static_cast<uint64_t *>(di.ptr), static_cast<float *>(ds.ptr), stream_);
-        NNDEX_HIP_CHECK(hipEventRecord(e1, stream_));
-        NNDEX_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0;
-        NNDEX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        ms /= reps;
-        report += "scan tiles_per_wave=" + std::to_string(m) + " grid=" +
-                  std::to_string(scan_grid()) + " us=" + std::to_string(ms * 1e3f) + "\n";
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_m = m;
+    uint32_t best_m = 0, best_f = kScanNonTemporal;
+    for (uint32_t f : {kScanNonTemporal, 0u, kScanNonTemporal | kScanContiguous, kScanContiguous}) {
+        for (uint32_t m : {2u, 3u, 4u, 6u, 8u, 12u, 16u, 20u, 24u, 32u, 48u}) {
+            if ((tiles + 4 * m - 1) / (4 * m) > ncu_ * 8) continue;
+            scan_tpw_ = m;
+            scan_flags_ = f;
+            auto ws = make_workspace(1, 10);
+            for (int i = 0; i < 3; ++i)
+                search_device(*ws, static_cast<const float *>(dq.ptr), 1, 10,
+                              static_cast<uint64_t *>(di.ptr), static_cast<float *>(ds.ptr),
+                              stream_);
+            NNDEX_HIP_CHECK(hipEventRecord(e0, stream_));
+            for (int i = 0; i < reps; ++i)
+                search_device(*ws, static_cast<const float *>(dq.ptr), 1, 10,
+                              static_cast<uint64_t *>(di.ptr), static_cast<float *>(ds.ptr),
+                              stream_);
+            NNDEX_HIP_CHECK(hipEventRecord(e1, stream_));
+            NNDEX_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            NNDEX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            report += "scan flags=" + std::to_string(f) + " tiles_per_wave=" + std::to_string(m) +
+                      " grid=" + std::to_string(scan_grid()) +
+                      " us=" + std::to_string(ms * 1e3f) + "\n";
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_m = m;
+                best_f = f;
+            }
         }
     }
+    scan_flags_ = best_f;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     scan_tpw_ = best_m;
     ws_.reset();  // grids changed; the host path re-creates its workspace
-    report += "chosen tiles_per_wave=" + std::to_string(best_m) + "\n";
+    report += "chosen flags=" + std::to_string(best_f) + " tiles_per_wave=" +
+              std::to_string(best_m) + "\n";
     return report;
 }
 
@@ -433,9 +449,11 @@ std::unique_ptr<Workspace> GpuIndex::make_workspace(size_t max_queries, size_t m
     ws->max_queries = max_queries;
     ws->max_k = max_k;
     ws->grid = scan_grid();
-    // QB * k < 2048 for every fused-scan variant.
-    ws->part_elems = static_cast<size_t>(2048) * ws->grid;
+    // QB * k < 2048 for every fused-scan variant; room for the largest grid.
+    const size_t gmax = static_cast<size_t>(ncu_) * 8;
+    ws->part_elems = 2048 * gmax;
     NNDEX_HIP_CHECK(hipMalloc(&ws->part, ws->part_elems * sizeof(uint64_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ws->part_count, 8 * gmax * sizeof(uint32_t)));
     NNDEX_HIP_CHECK(hipMalloc(&ws->theta, 8 * sizeof(unsigned long long)));
     NNDEX_HIP_CHECK(hipMalloc(&ws->ticket, sizeof(uint32_t)));
     NNDEX_HIP_CHECK(hipMemset(ws->theta, 0, 8 * sizeof(unsigned long long)));
@@ -465,11 +483,13 @@ void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k
         a.nvalid = nvalid;
         a.k = static_cast<uint32_t>(k);
         a.part = ws.part;
+        a.part_count = ws.part_count;
         a.theta = ws.theta;
         a.ticket = ws.ticket;
         a.out_idx = d_idx + q0 * k;
         a.out_sim = d_sim + q0 * k;
         a.out_stride = static_cast<uint32_t>(k);
+        a.flags = scan_flags_;
         const uint32_t grid = scan_grid();
         if (static_cast<size_t>(grid) * 2048 > ws.part_elems)
             throw Error(NNDEX_ERR_INVALID_ARGUMENT,
@@ -771,10 +791,12 @@ nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char
     });
 }
 
-nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave) {
+nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave,
+                                        uint32_t scan_flags) {
     return guard([&] {
         if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
         index->impl->set_scan_tiles_per_wave(scan_tiles_per_wave);
+        index->impl->set_scan_flags(scan_flags);
     });
 }
 

@@ -34,6 +34,7 @@ struct Workspace {
     uint32_t grid = 0;
     uint64_t *part = nullptr;
     size_t part_elems = 0;
+    uint32_t *part_count = nullptr;  // [8][part_elems / 2048]
     unsigned long long *theta = nullptr;  // [8]
     uint32_t *ticket = nullptr;
     // batched MFMA path
@@ -125,6 +126,7 @@ class GpuIndex {
     uint32_t scan_grid() const;
     // Tiles of 64 rows each wave of the fused scan owns (0 = default).
     void set_scan_tiles_per_wave(uint32_t m) { scan_tpw_ = m; }
+    void set_scan_flags(uint32_t f) { scan_flags_ = f; }
     uint32_t scan_tiles_per_wave() const;
     // Time the fused scan over candidate launch shapes and keep the fastest;
     // returns a one-line report per candidate.
@@ -144,11 +146,13 @@ class GpuIndex {
     int device_ = 0;
     uint32_t ncu_ = 0;
     uint32_t scan_tpw_ = 0;
+    uint32_t scan_flags_ = kScanNonTemporal;
     size_t rows_ = 0, pdims_ = 0, d4_ = 0;
     bool approx_ = false, enable_cache_ = true;
     float4 *x_ = nullptr;
     uint4 *xbf16_ = nullptr;
     unsigned long long *xmax_sq_ = nullptr;
+    unsigned long long *xerr_sq_ = nullptr;  // [0] unused, [1] BF16 rounding error
     std::mutex lazy_mu_;
     std::vector<RowSample> samples_;
     const RowSample &sample_for(uint32_t stride, bool bf16, hipStream_t s);

@@ -7,6 +7,10 @@
 
 namespace nndex {
 
+// Launch-shape flags of the fused exact scan.
+constexpr uint32_t kScanNonTemporal = 1;  // streaming (non-temporal) loads
+constexpr uint32_t kScanContiguous = 2;   // contiguous tile runs per wave
+
 // Arguments of the fused exact scan (exact.hip).
 struct ScanArgs {
     const float4 *x;              // tiled matrix, rows padded to 64
@@ -17,11 +21,13 @@ struct ScanArgs {
     uint32_t nvalid;              // queries actually present (<= QB); later ones repeat the last
     uint32_t k;                   // neighbours per query (1 <= k <= rows)
     uint64_t *part;               // [QB][grid][k] block lists
+    uint32_t *part_count;         // [QB][grid] live keys per block list
     unsigned long long *theta;    // [QB] device-wide thresholds (0 between launches)
     uint32_t *ticket;             // block-completion counter (0 between launches)
     uint64_t *out_idx;            // [QB][out_stride]
     float *out_sim;               // [QB][out_stride]
     uint32_t out_stride;
+    uint32_t flags;               // kScan* launch-shape flags
 };
 
 void launch_to_tiled(const float *src, uint32_t rows, uint32_t dims, uint32_t d4, float4 *dst,
