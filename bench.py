@@ -13,7 +13,13 @@ Also reported:
                 against the 8 TB/s HBM3E peak;
   cpu_baseline  the CPU oracle's restatement of the reference exact path
                 (ParallelFold strategy, all threads) on a bounded sample;
-  batch         QPS at batch sizes 16 / 256 / 4096 (exact FP32).
+  batch         QPS at batch sizes 16 / 256 / 4096 (exact FP32) and the BF16
+                scoring + exact FP32 rerank form at 4096;
+  ivf           the IVF form: 10,000,000 x 128 Gaussian mixture (4,096
+                centres, spread 0.3), k-means with nlist 4,096 and 20
+                iterations built on the GPU, 1,024 queries at nprobe 8 / 32 /
+                128 with k=10: build time, QPS, recall@10 against the exact
+                search and the list scan's achieved bandwidth.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -77,6 +83,78 @@ def elapsed(a, b):
     return ms.value
 
 
+def ivf_bench(args, k: int = 10) -> dict:
+    """The IVF form on a resident index (see the module docstring)."""
+    L = _lib.lib()
+    rows, dims, nlist, nq = args.ivf_rows, 128, args.ivf_nlist, args.ivf_queries
+    t0 = time.time()
+    centres = synth.mixture_centres(dims, 4096, seed=7)
+    base = synth.gaussian_mixture(rows, dims, centres, 0.3, 42)
+    queries = synth.gaussian_mixture(nq, dims, centres, 0.3, 43)
+    log(f"ivf workload {rows}x{dims} generated in {time.time() - t0:.1f}s")
+    h = ctypes.c_void_p()
+    check(L.nndex_gpu_index_new(base.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rows, dims,
+                                1, 0, -1, ctypes.byref(h)))
+    del base
+    check(L.nndex_gpu_index_set_ivf_params(h, nlist, 0, args.ivf_iters))
+    info = _lib.IvfInfo()
+    t0 = time.perf_counter()
+    check(L.nndex_gpu_index_build_ivf(h, ctypes.byref(info)))
+    build_s = time.perf_counter() - t0
+    log(f"ivf build: {build_s:.2f} s (device {info.build_ms:.0f} ms), nlist {info.nlist}, "
+        f"iters {info.iters}, rechecked rows {info.rechecked}")
+    # cluster sizes and centroids for the algorithmic byte count of the list scan
+    offs = np.zeros(nlist + 1, np.uint64)
+    cent = np.zeros((nlist, dims), np.float32)
+    check(L.nndex_gpu_index_ivf_export(h, cent.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                       None, None))
+    sizes = np.diff(offs.astype(np.int64))
+    cscore = queries @ cent.T
+    ws = ctypes.c_void_p()
+    check(L.nndex_workspace_new(h, nq, 128, ctypes.byref(ws)))
+    stream = ctypes.c_void_p()
+    check(L.nndex_stream_create(ctypes.byref(stream)))
+    dq = Dev(queries.nbytes).put(queries)
+    ei, es = Dev(nq * k * 8), Dev(nq * k * 4)
+    check(L.nndex_gpu_index_search_device(h, ws, dq.ptr, nq, k, ei.ptr, es.ptr, stream))
+    check(L.nndex_stream_synchronize(stream))
+    exact = ei.get((nq, k), np.uint64)
+    di, ds_, dl = Dev(nq * k * 8), Dev(nq * k * 4), Dev(nq * 8)
+    res = {"workload": f"{rows}x{dims} Gaussian mixture (4096 centres, spread 0.3, "
+                       f"unit-normalised), {nq} queries, k={k}",
+           "nlist": int(info.nlist), "iters": int(info.iters),
+           "build_s": round(build_s, 3), "rechecked_rows": int(info.rechecked)}
+    for nprobe in (8, 32, 128):
+        def run():
+            check(L.nndex_gpu_index_search_ivf_device(h, ws, dq.ptr, nq, k, nprobe, di.ptr,
+                                                      ds_.ptr, dl.ptr, stream))
+        run()
+        check(L.nndex_stream_synchronize(stream))
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        check(L.nndex_stream_synchronize(stream))
+        dt = (time.perf_counter() - t0) / reps
+        got = di.get((nq, k), np.uint64)
+        recall = float(np.mean([len(set(got[i].tolist()) & set(exact[i].tolist())) / k
+                                for i in range(nq)]))
+        probed = np.argpartition(-cscore, nprobe - 1, axis=1)[:, :nprobe]
+        scanned = int(sizes[probed].sum())
+        gbs = scanned * dims * 4 / dt / 1e9
+        res[f"nprobe_{nprobe}"] = {"qps": round(nq / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
+                                   f"recall_at_{k}": round(recall, 5),
+                                   "rows_scanned_per_query": round(scanned / nq, 1),
+                                   "list_scan_gbs": round(gbs, 1)}
+        log(f"ivf nprobe {nprobe}: {nq / dt:.0f} QPS ({dt * 1e3:.2f} ms), recall@{k} "
+            f"{recall:.4f}, {scanned / nq:.0f} rows/query, {gbs:.0f} GB/s")
+    L.nndex_stream_destroy(stream)
+    L.nndex_workspace_free(ws)
+    L.nndex_gpu_index_free(h)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +172,13 @@ def main():
                     help="fix the scan launch shape instead of autotuning")
     ap.add_argument("--scan-flags", type=int, default=1,
                     help="scan flags with --tiles-per-wave (1 non-temporal, 2 contiguous)")
+    ap.add_argument("--no-ivf", action="store_true")
+    ap.add_argument("--ivf-only", action="store_true",
+                    help="run only the IVF form and print its record")
+    ap.add_argument("--ivf-rows", type=int, default=10_000_000)
+    ap.add_argument("--ivf-nlist", type=int, default=4096)
+    ap.add_argument("--ivf-iters", type=int, default=20)
+    ap.add_argument("--ivf-queries", type=int, default=1024)
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: copy calibration + a few searches only")
     args = ap.parse_args()
@@ -103,6 +188,9 @@ def main():
         sys.exit(2)
 
     L = _lib.lib()
+    if args.ivf_only:
+        print(json.dumps({"ivf": ivf_bench(args)}), flush=True)
+        return
     rows, dims, k = args.rows, args.dims, args.k
     t0 = time.time()
     base, bank = synth.mixture_workload(rows=rows, dims=dims, n_queries=4096)
@@ -288,6 +376,11 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    L.nndex_workspace_free(ws)
+    L.nndex_gpu_index_free(h)
+    del base
+    ivf = None if args.no_ivf else ivf_bench(args)
+
     out = {
         "metric": METRIC,
         "value": round(qps, 1),
@@ -315,10 +408,9 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
         "autotune": tune.strip().splitlines()[-1] if tune else None,
         "batch": batch,
+        "ivf": ivf,
     }
     print(json.dumps(out), flush=True)
-    L.nndex_workspace_free(ws)
-    L.nndex_gpu_index_free(h)
 
 
 if __name__ == "__main__":

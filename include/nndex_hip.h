@@ -178,6 +178,52 @@ nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char
 nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave,
                                         uint32_t scan_flags);
 
+/* ---- IVF prefilter (src/approx.rs, used by src/gpu.rs:435-462) ---------- */
+
+/* Built IVF index summary.  `rechecked` counts the assignment rows (summed
+ * over iterations) whose top-two MFMA scores were too close to separate and
+ * were re-evaluated in the reference dot order. */
+typedef struct nndex_ivf_info {
+    uint64_t rows;
+    uint64_t padded_dims;
+    uint64_t nlist;
+    uint64_t iters;
+    uint64_t rechecked;
+    double build_ms;
+} nndex_ivf_info;
+
+/* IVF parameters (replaces the constants of src/approx.rs: choose_nlist
+ * :305-311, KMEANS_MAX_ITERS :28, choose_nprobe :318-336); 0 keeps the
+ * reference's rule for that field.  nlist and iters apply to the next build
+ * only, so they must be set before the first approximate search. */
+nndex_status nndex_gpu_index_set_ivf_params(nndex_gpu_index *index, size_t nlist, size_t nprobe,
+                                            size_t iters);
+/* Build the IVF index now on the device (ApproxIndex::build,
+ * src/approx.rs:46-100; the reference builds lazily on the first approximate
+ * search, src/gpu.rs:435-441, which nndex_gpu_index_search also does).  A
+ * second call returns the existing build.  Works for any index; searches
+ * with approx = 1 use it only if the index was created with approx = 1. */
+nndex_status nndex_gpu_index_build_ivf(nndex_gpu_index *index, nndex_ivf_info *out);
+/* Copy the built IVF index to host memory; any pointer may be NULL.
+ * centroids: nlist x padded_dims; offsets: nlist + 1 (cluster_offsets);
+ * original_indices: rows (reordered position -> row); assignments: rows. */
+nndex_status nndex_gpu_index_ivf_export(nndex_gpu_index *index, float *centroids,
+                                        uint64_t *offsets, uint64_t *original_indices,
+                                        uint64_t *assignments);
+/* Device-resident approximate search (candidate_indices + rerank,
+ * src/approx.rs:110-138 and src/gpu.rs:448-462): the nprobe (0 = the
+ * reference's choose_nprobe for k) clusters whose centroids score highest,
+ * then every row of those clusters rescored exactly into a top-k.  Writes
+ * query_rows x k results and device_lens[q] = results found (< k when the
+ * probed clusters hold fewer than k rows).  Builds the IVF index first if
+ * needed.  Synchronises `stream` once when query_rows > 8. */
+nndex_status nndex_gpu_index_search_ivf_device(nndex_gpu_index *index, nndex_workspace *ws,
+                                               const float *device_queries, size_t query_rows,
+                                               size_t k, size_t nprobe,
+                                               uint64_t *device_indices,
+                                               float *device_similarities,
+                                               uint64_t *device_lens, void *stream);
+
 /* ======================================================================== */
 /* 3. Front-end NNdex (src/lib.rs) in C++                                   */
 /* ======================================================================== */

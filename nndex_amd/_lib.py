@@ -93,6 +93,19 @@ class GpuDeviceInfo(ctypes.Structure):
         }
 
 
+class IvfInfo(ctypes.Structure):
+    """nndex_ivf_info: summary of a built IVF index."""
+
+    _fields_ = [
+        ("rows", ctypes.c_uint64),
+        ("padded_dims", ctypes.c_uint64),
+        ("nlist", ctypes.c_uint64),
+        ("iters", ctypes.c_uint64),
+        ("rechecked", ctypes.c_uint64),
+        ("build_ms", ctypes.c_double),
+    ]
+
+
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
 _I32 = ctypes.c_int32
@@ -121,6 +134,10 @@ SIGNATURES = {
                                                   _P]),
     "nndex_gpu_index_autotune": (_ST, [_P, _I32, ctypes.c_char_p, _SZ]),
     "nndex_gpu_index_set_launch": (_ST, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    "nndex_gpu_index_set_ivf_params": (_ST, [_P, _SZ, _SZ, _SZ]),
+    "nndex_gpu_index_build_ivf": (_ST, [_P, ctypes.POINTER(IvfInfo)]),
+    "nndex_gpu_index_ivf_export": (_ST, [_P, _FP, _U64P, _U64P, _U64P]),
+    "nndex_gpu_index_search_ivf_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _P, _P]),
     "nndex_device_copy": (_ST, [_P, _P, _SZ, _P]),
     "nndex_new": (_ST, [_FP, _SZ, _SZ, _SZ, ctypes.POINTER(IndexOptions), ctypes.POINTER(_P)]),
     "nndex_free": (None, [_P]),

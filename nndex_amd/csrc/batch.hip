@@ -200,9 +200,10 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // group g = lane/16 owns units 4s+g), NT 16-query MFMA tiles per query chunk.
 // The next chunk's queries and thresholds are fetched into registers while
 // the current chunk runs on the MFMA units, then written to the other LDS
-// buffer.  Mode 0 appends candidates, mode 1 writes scores.
+// buffer.  Mode 0 appends candidates, mode 1 writes scores; the AM variant
+// (mode 2) keeps each row's two best queries and writes the row argmax.
 // ---------------------------------------------------------------------------
-template <typename E, int NW, int RW, int MAXS, int NT>
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM>
 __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
     constexpr uint32_t QC = 16 * NT;     // queries per chunk
@@ -271,6 +272,13 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
                 xr[m][s] = (in && c < dch) ? base[(size_t)c * 64] : make_uint4(0, 0, 0, 0);
             }
         }
+        uint64_t best[MT][4], second[MT][4];  // row argmax variant only
+        if constexpr (AM) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) best[m][v] = second[m][v] = 0;
+        }
         __syncthreads();  // previous block tile done with both buffers
         fetch(0);
         commit(0);
@@ -320,7 +328,16 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
-                        if (a.mode == 0) {
+                        if constexpr (AM) {
+                            const uint64_t key =
+                                qi < a.nq ? make_key(E::total(acc[m][t], v), qi) : 0;
+                            if (key > best[m][v]) {
+                                second[m][v] = best[m][v];
+                                best[m][v] = key;
+                            } else if (key > second[m][v]) {
+                                second[m][v] = key;
+                            }
+                        } else if (a.mode == 0) {
                             const uint64_t key = make_key(E::total(acc[m][t], v), row);
                             const bool pass = qi < a.nq && row < a.rows && key >= th_cur[t];
                             if (__ballot(pass) != 0 && pass) {
@@ -339,6 +356,28 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
                 for (int t = 0; t < NT; ++t) th_cur[t] = th_next[t];
             }
             __syncthreads();
+        }
+        if constexpr (AM) {
+            // top two over the 16 lanes that hold the same rows (one query
+            // each); a pair closer than tol goes to the exact re-check
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    uint64_t b = best[m][v], c = second[m][v];
+                    for (int off = 1; off < 16; off <<= 1) {
+                        const uint64_t ob = __shfl_xor(b, off), oc = __shfl_xor(c, off);
+                        const uint64_t lo = ob < b ? ob : b, hc = oc > c ? oc : c;
+                        b = ob > b ? ob : b;
+                        c = lo > hc ? lo : hc;
+                    }
+                    const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
+                    if (l16 == 0 && row < a.rows) {
+                        a.assign[row] = key_row(b);
+                        if (c != 0 && !(key_sim(b) - key_sim(c) > a.tol))
+                            a.recheck[atomicAdd(a.recheck_count, 1u)] = row;
+                    }
+                }
         }
     }
 }
@@ -603,10 +642,10 @@ void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, bool bf16,
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-template <typename E, int NW, int RW, int MAXS, int NT>
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
 static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     const size_t lds = 2ull * MAXS * 4 * (16 * NT + 1) * sizeof(uint4);
-    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT>;
+    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM>;
     NNDEX_HIP_CHECK(hipFuncSetAttribute((const void *)fn,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(fn, dim3(grid), dim3(NW * 64), lds, s, a);
@@ -632,6 +671,9 @@ void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream
                                            "supported by the MFMA scan");
 #define NNDEX_MFMA_CASE(STEPS, F32_NT, BF_RW, BF_NT)                                   \
     case STEPS:                                                                        \
+        if (a.mode == 2 && !bf16)                                                      \
+            return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT, true>(a, grid, s);        \
+        if (a.mode == 2) break;                                                        \
         if (bf16)                                                                      \
             return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT>(a, grid, s);           \
         return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT>(a, grid, s);

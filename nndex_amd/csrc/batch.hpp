@@ -15,13 +15,20 @@ struct MfmaScanArgs {
     const uint4 *q;            // queries, row-major, q_stride units per query
     uint32_t q_stride;
     uint32_t nq;
-    uint32_t mode;             // 0: append candidates, 1: write scores
+    uint32_t mode;             // 0: append candidates, 1: write scores, 2: row argmax
     const unsigned long long *thresh;  // [nq] key thresholds (mode 0)
     uint64_t *cand;            // [nq][capq] candidate keys (mode 0)
     uint32_t *cand_count;      // [nq] (mode 0; may exceed capq on overflow)
     uint32_t capq;
     float *out;                // [nq][ldo] scores (mode 1)
     uint32_t ldo;
+    // mode 2: per row, the best query (ties -> lowest id); a row whose best
+    // and second-best approximate scores are within `tol` is appended to
+    // `recheck` for an exact re-evaluation
+    uint32_t *assign;          // [rows]
+    float tol;
+    uint32_t *recheck;         // [rows]
+    uint32_t *recheck_count;
 };
 
 struct RerankArgs {

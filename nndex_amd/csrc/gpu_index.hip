@@ -1,6 +1,7 @@
 // GpuIndex implementation and the backend C ABI (replaces src/gpu.rs).
 #include "gpu_index.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -46,7 +47,7 @@ PinnedBuffer::~PinnedBuffer() {
 Workspace::~Workspace() {
     for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
                     (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
-                    (void *)flags})
+                    (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part})
         if (p) (void)hipFree(p);
 }
 
@@ -75,6 +76,18 @@ void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t cap
     if (nq * capq > b_cand) {
         regrow(cand, nq * capq);
         b_cand = nq * capq;
+    }
+}
+
+void Workspace::reserve_ivf(size_t nq, size_t nprobe, size_t k) {
+    if (nq * nprobe > ivf_probe_elems) {
+        regrow(ivf_probe, nq * nprobe);
+        regrow(ivf_probe_sim, nq * nprobe);
+        ivf_probe_elems = nq * nprobe;
+    }
+    if (nq * nprobe * k > ivf_part_elems) {
+        regrow(ivf_part, nq * nprobe * k);
+        ivf_part_elems = nq * nprobe * k;
     }
 }
 
@@ -516,8 +529,11 @@ uint64_t GpuIndex::query_hash(const float *values, size_t n, size_t k, size_t qu
 }
 
 std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size_t query_rows,
-                                                      size_t k, size_t raw_dims, bool normalize) {
+                                                      size_t k, size_t raw_dims, bool normalize,
+                                                      bool approx) {
     NNDEX_HIP_CHECK(hipSetDevice(device_));
+    // dispatch_*_or_approx (src/gpu.rs:359-367, :404-432)
+    const IvfIndex *ivf = approx ? approx_index() : nullptr;
     if (!ws_ || ws_->max_k < k) ws_ = make_workspace(8, k < 10 ? 10 : k);
     const size_t qbytes = query_rows * pdims_ * sizeof(float);
     const size_t in_bytes = query_rows * (raw_dims ? raw_dims : pdims_) * sizeof(float);
@@ -549,9 +565,20 @@ std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size
         NNDEX_HIP_CHECK(
             hipMemcpyAsync(d_q_.ptr, h_q_.ptr, qbytes, hipMemcpyHostToDevice, stream_));
     }
-    search_exact_any(*ws_, static_cast<const float *>(d_q_.ptr), query_rows, k,
-                     static_cast<uint64_t *>(d_idx_.ptr), static_cast<float *>(d_sim_.ptr),
-                     stream_);
+    std::vector<uint64_t> lens;
+    if (ivf) {
+        d_len_.reserve(query_rows * sizeof(uint64_t));
+        search_ivf_device(*ws_, static_cast<const float *>(d_q_.ptr), query_rows, k, 0,
+                          static_cast<uint64_t *>(d_idx_.ptr), static_cast<float *>(d_sim_.ptr),
+                          static_cast<uint64_t *>(d_len_.ptr), stream_);
+        lens.resize(query_rows);
+        NNDEX_HIP_CHECK(hipMemcpyAsync(lens.data(), d_len_.ptr, query_rows * sizeof(uint64_t),
+                                       hipMemcpyDeviceToHost, stream_));
+    } else {
+        search_exact_any(*ws_, static_cast<const float *>(d_q_.ptr), query_rows, k,
+                         static_cast<uint64_t *>(d_idx_.ptr), static_cast<float *>(d_sim_.ptr),
+                         stream_);
+    }
     NNDEX_HIP_CHECK(hipMemcpyAsync(h_idx_.ptr, d_idx_.ptr, obytes * sizeof(uint64_t),
                                    hipMemcpyDeviceToHost, stream_));
     NNDEX_HIP_CHECK(hipMemcpyAsync(h_sim_.ptr, d_sim_.ptr, obytes * sizeof(float),
@@ -562,8 +589,9 @@ std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size
     const uint64_t *hi = static_cast<const uint64_t *>(h_idx_.ptr);
     const float *hs = static_cast<const float *>(h_sim_.ptr);
     for (size_t q = 0; q < query_rows; ++q) {
-        out[q].resize(k);
-        for (size_t i = 0; i < k; ++i) out[q][i] = {hi[q * k + i], hs[q * k + i]};
+        const size_t n = ivf ? static_cast<size_t>(lens[q]) : k;
+        out[q].resize(n);
+        for (size_t i = 0; i < n; ++i) out[q][i] = {hi[q * k + i], hs[q * k + i]};
     }
     return out;
 }
@@ -575,12 +603,12 @@ std::vector<Neighbor> GpuIndex::search(const float *query, size_t k, bool approx
         uint64_t key = query_hash(query, pdims_, k, 1, approx);
         auto it = cache_single_.find(key);
         if (it != cache_single_.end()) return it->second;
-        auto res = std::move(run_host(query, 1, capped, 0, false)[0]);
+        auto res = std::move(run_host(query, 1, capped, 0, false, approx)[0]);
         if (cache_single_.size() >= kSingleCacheCapacity) cache_single_.clear();
         cache_single_.emplace(key, res);
         return res;
     }
-    return std::move(run_host(query, 1, capped, 0, false)[0]);
+    return std::move(run_host(query, 1, capped, 0, false, approx)[0]);
 }
 
 std::vector<std::vector<Neighbor>> GpuIndex::search_batch(const float *queries, size_t query_rows,
@@ -592,12 +620,12 @@ std::vector<std::vector<Neighbor>> GpuIndex::search_batch(const float *queries, 
         uint64_t key = query_hash(queries, query_rows * pdims_, k, query_rows, approx);
         auto it = cache_batch_.find(key);
         if (it != cache_batch_.end()) return it->second;
-        auto res = run_host(queries, query_rows, capped, 0, false);
+        auto res = run_host(queries, query_rows, capped, 0, false, approx);
         if (cache_batch_.size() >= kBatchCacheCapacity) cache_batch_.clear();
         cache_batch_.emplace(key, res);
         return res;
     }
-    return run_host(queries, query_rows, capped, 0, false);
+    return run_host(queries, query_rows, capped, 0, false, approx);
 }
 
 std::vector<std::vector<Neighbor>> GpuIndex::search_batch_raw(const float *queries,
@@ -614,12 +642,115 @@ std::vector<std::vector<Neighbor>> GpuIndex::search_batch_raw(const float *queri
                        (normalize ? 0x9e3779b97f4a7c15ull : 0ull);
         auto it = cache_batch_.find(key);
         if (it != cache_batch_.end()) return it->second;
-        auto res = run_host(queries, query_rows, capped, dims, normalize);
+        auto res = run_host(queries, query_rows, capped, dims, normalize, approx);
         if (cache_batch_.size() >= kBatchCacheCapacity) cache_batch_.clear();
         cache_batch_.emplace(key, res);
         return res;
     }
-    return run_host(queries, query_rows, capped, dims, normalize);
+    return run_host(queries, query_rows, capped, dims, normalize, approx);
+}
+
+// ---------------------------------------------------------------------------
+// IVF prefilter
+// ---------------------------------------------------------------------------
+
+std::unique_ptr<GpuIndex> GpuIndex::from_device_rows(const float *d_rows, size_t rows,
+                                                     size_t padded_dims, int device) {
+    check_rows(rows);
+    std::unique_ptr<GpuIndex> g(new GpuIndex());
+    g->init_device(device);
+    g->rows_ = rows;
+    g->pdims_ = padded_dims;
+    g->d4_ = padded_dims / 4;
+    g->enable_cache_ = false;
+    const size_t tiles = padded_rows(rows) / kTileRows;
+    NNDEX_HIP_CHECK(hipMalloc(&g->x_, tiles * g->d4_ * kTileRows * sizeof(float4)));
+    DeviceBuffer flags;
+    flags.reserve(sizeof(uint32_t));
+    launch_zero_u32(static_cast<uint32_t *>(flags.ptr), 1, g->stream_);
+    launch_to_tiled(d_rows, static_cast<uint32_t>(rows), static_cast<uint32_t>(padded_dims),
+                    static_cast<uint32_t>(g->d4_), g->x_, static_cast<uint32_t *>(flags.ptr),
+                    g->stream_);
+    NNDEX_HIP_CHECK(hipMalloc(&g->xmax_sq_, sizeof(unsigned long long)));
+    NNDEX_HIP_CHECK(hipMemsetAsync(g->xmax_sq_, 0, sizeof(unsigned long long), g->stream_));
+    launch_max_norm(g->x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(g->d4_),
+                    g->xmax_sq_, g->stream_);
+    NNDEX_HIP_CHECK(hipStreamSynchronize(g->stream_));
+    return g;
+}
+
+void GpuIndex::set_ivf_params(size_t nlist, size_t nprobe, size_t iters) {
+    std::lock_guard<std::mutex> l(ivf_mu_);
+    if (nlist > rows_ || nlist == 1 || nlist > 16384)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "nlist must be 0 or in [2, min(rows, 16384)]");
+    ivf_nlist_ = nlist;
+    ivf_nprobe_ = nprobe;
+    ivf_iters_ = iters;
+}
+
+const IvfIndex &GpuIndex::build_ivf() {
+    std::lock_guard<std::mutex> l(ivf_mu_);
+    if (ivf_) return *ivf_;
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    std::unique_ptr<IvfIndex> ivf(new IvfIndex());
+    const size_t nlist = ivf_nlist_ ? ivf_nlist_ : ivf_choose_nlist(rows_);
+    const size_t iters = ivf_iters_ ? ivf_iters_ : kKmeansMaxIters;
+    if (nlist < 2) {
+        // ApproxIndex::build's single-list form (approx.rs:50-60): never used
+        // for searching (provides_speedup is false), kept for the export.
+        ivf->rows = static_cast<uint32_t>(rows_);
+        ivf->pdims = static_cast<uint32_t>(pdims_);
+        ivf->nlist = 1;
+        ivf_ = std::move(ivf);
+        return *ivf_;
+    }
+    unsigned long long bits = 0;
+    NNDEX_HIP_CHECK(hipMemcpy(&bits, xmax_sq_, sizeof(bits), hipMemcpyDeviceToHost));
+    const double x_max = std::sqrt(__builtin_bit_cast(double, bits));
+    ivf_build(*ivf, x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(pdims_),
+              static_cast<uint32_t>(nlist), static_cast<uint32_t>(iters), x_max, ncu_, stream_);
+    ivf->cindex = from_device_rows(ivf->centroids, nlist, pdims_, device_);
+    ivf_ = std::move(ivf);
+    return *ivf_;
+}
+
+const IvfIndex *GpuIndex::approx_index() {
+    if (!approx_) return nullptr;
+    const IvfIndex &ivf = build_ivf();
+    return ivf.provides_speedup() ? &ivf : nullptr;
+}
+
+void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size_t k,
+                                 size_t nprobe, uint64_t *d_idx, float *d_sim, uint64_t *d_len,
+                                 hipStream_t s) {
+    if (k == 0 || k > rows_) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+    const IvfIndex &ivf = build_ivf();
+    if (ivf.nlist < 2)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "the IVF index has a single list (rows < 256)");
+    if (nq == 0) return;
+    if (nprobe == 0) nprobe = ivf_nprobe_;
+    if (nprobe == 0) nprobe = ivf_choose_nprobe(ivf.nlist, k, rows_);
+    if (nprobe > ivf.nlist) nprobe = ivf.nlist;
+    ws.reserve_ivf(nq, nprobe, k);
+    if (!ws.sub || ws.sub->max_k < nprobe)
+        ws.sub = ivf.cindex->make_workspace(8, nprobe < 10 ? 10 : nprobe);
+    // candidate_indices (approx.rs:110-138): top-nprobe centroids by score
+    ivf.cindex->search_exact_any(*ws.sub, dq, nq, nprobe, ws.ivf_probe, ws.ivf_probe_sim, s);
+    IvfScanArgs a{};
+    a.x = ivf.xr;
+    a.d4 = static_cast<uint32_t>(d4_);
+    a.q = dq;
+    a.probe = ws.ivf_probe;
+    a.probe_stride = static_cast<uint32_t>(nprobe);
+    a.nprobe = static_cast<uint32_t>(nprobe);
+    a.offsets = ivf.offsets;
+    a.orig = ivf.orig;
+    a.k = static_cast<uint32_t>(k);
+    a.part = ws.ivf_part;
+    a.out_idx = d_idx;
+    a.out_sim = d_sim;
+    a.out_len = d_len;
+    ivf_scan(a, static_cast<uint32_t>(nq), s);
 }
 
 }  // namespace nndex
@@ -797,6 +928,81 @@ nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_ti
         if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
         index->impl->set_scan_tiles_per_wave(scan_tiles_per_wave);
         index->impl->set_scan_flags(scan_flags);
+    });
+}
+
+nndex_status nndex_gpu_index_set_ivf_params(nndex_gpu_index *index, size_t nlist, size_t nprobe,
+                                            size_t iters) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        index->impl->set_ivf_params(nlist, nprobe, iters);
+    });
+}
+
+nndex_status nndex_gpu_index_build_ivf(nndex_gpu_index *index, nndex_ivf_info *out) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        const IvfIndex &ivf = index->impl->build_ivf();
+        if (out) {
+            out->rows = ivf.rows;
+            out->padded_dims = ivf.pdims;
+            out->nlist = ivf.nlist;
+            out->iters = ivf.iters;
+            out->rechecked = ivf.rechecked;
+            out->build_ms = ivf.build_ms;
+        }
+    });
+}
+
+nndex_status nndex_gpu_index_ivf_export(nndex_gpu_index *index, float *centroids,
+                                        uint64_t *offsets, uint64_t *original_indices,
+                                        uint64_t *assignments) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        const IvfIndex &ivf = index->impl->build_ivf();
+        NNDEX_HIP_CHECK(hipSetDevice(index->impl->device()));
+        const size_t rows = ivf.rows, nl = ivf.nlist;
+        auto widen = [](const uint32_t *d, size_t n, uint64_t *out) {
+            std::vector<uint32_t> h(n);
+            NNDEX_HIP_CHECK(hipMemcpy(h.data(), d, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) out[i] = h[i];
+        };
+        if (nl < 2) {  // single list: identity order (approx.rs:50-60)
+            if (offsets) {
+                offsets[0] = 0;
+                offsets[1] = rows;
+            }
+            for (size_t i = 0; i < rows; ++i) {
+                if (original_indices) original_indices[i] = i;
+                if (assignments) assignments[i] = 0;
+            }
+            return;
+        }
+        if (centroids)
+            NNDEX_HIP_CHECK(hipMemcpy(centroids, ivf.centroids, nl * ivf.pdims * sizeof(float),
+                                      hipMemcpyDeviceToHost));
+        if (offsets) widen(ivf.offsets, nl + 1, offsets);
+        if (original_indices) widen(ivf.orig, rows, original_indices);
+        if (assignments) widen(ivf.assign, rows, assignments);
+    });
+}
+
+nndex_status nndex_gpu_index_search_ivf_device(nndex_gpu_index *index, nndex_workspace *ws,
+                                               const float *device_queries, size_t query_rows,
+                                               size_t k, size_t nprobe,
+                                               uint64_t *device_indices,
+                                               float *device_similarities,
+                                               uint64_t *device_lens, void *stream) {
+    return guard([&] {
+        if (!index || !ws || !device_queries || !device_indices || !device_similarities ||
+            !device_lens)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (ws->owner != index->impl.get())
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "workspace belongs to another index");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        index->impl->search_ivf_device(*ws->impl, device_queries, query_rows, k, nprobe,
+                                       device_indices, device_similarities, device_lens,
+                                       static_cast<hipStream_t>(stream));
     });
 }
 

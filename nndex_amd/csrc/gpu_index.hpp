@@ -13,6 +13,7 @@
 
 #include "batch.hpp"
 #include "common.hpp"
+#include "ivf.hpp"
 #include "kernels.hpp"
 
 namespace nndex {
@@ -47,6 +48,14 @@ struct Workspace {
     uint32_t *cand_count = nullptr;
     uint8_t *flags = nullptr;
     void reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq);
+    // IVF search: probed cluster ids and scores, per-list keys, and the
+    // centroid index's own workspace
+    size_t ivf_probe_elems = 0, ivf_part_elems = 0;
+    uint64_t *ivf_probe = nullptr;
+    float *ivf_probe_sim = nullptr;
+    uint64_t *ivf_part = nullptr;
+    std::unique_ptr<Workspace> sub;
+    void reserve_ivf(size_t nq, size_t nprobe, size_t k);
     ~Workspace();
 };
 
@@ -122,6 +131,24 @@ class GpuIndex {
     bool mfma_supported(Precision p) const;
     static size_t default_overfetch(size_t k, Precision p);
 
+    // ---- IVF prefilter (src/approx.rs through src/gpu.rs:435-462) ----
+    // 0 keeps the reference's rule (choose_nlist, KMEANS_MAX_ITERS,
+    // choose_nprobe); nlist and iters apply to the next build.
+    void set_ivf_params(size_t nlist, size_t nprobe, size_t iters);
+    // The IVF index, built now if needed (any index).
+    const IvfIndex &build_ivf();
+    // approx_index (src/gpu.rs:435-441): the IVF index for approximate
+    // searches, built on first use, or nullptr when the index was created
+    // without approx or IVF would not speed it up.
+    const IvfIndex *approx_index();
+    // Enqueue an IVF search of device-resident queries (builds first if
+    // needed); lens[q] = results found.
+    void search_ivf_device(Workspace &ws, const float *dq, size_t nq, size_t k, size_t nprobe,
+                           uint64_t *d_idx, float *d_sim, uint64_t *d_len, hipStream_t s);
+    // An index over rows already on the device (row-major, padded dims).
+    static std::unique_ptr<GpuIndex> from_device_rows(const float *d_rows, size_t rows,
+                                                      size_t padded_dims, int device);
+
     std::unique_ptr<Workspace> make_workspace(size_t max_queries, size_t max_k) const;
     uint32_t scan_grid() const;
     // Tiles of 64 rows each wave of the fused scan owns (0 = default).
@@ -141,7 +168,8 @@ class GpuIndex {
                         bool approx) const;
     // raw_dims == 0: queries are normalised and padded already.
     std::vector<std::vector<Neighbor>> run_host(const float *queries, size_t query_rows,
-                                                size_t k, size_t raw_dims, bool normalize);
+                                                size_t k, size_t raw_dims, bool normalize,
+                                                bool approx);
 
     int device_ = 0;
     uint32_t ncu_ = 0;
@@ -161,7 +189,10 @@ class GpuIndex {
     nndex_gpu_device_info info_{};
     std::mutex mu_;
     std::unique_ptr<Workspace> ws_;
-    DeviceBuffer d_q_, d_raw_, d_idx_, d_sim_, d_flags_;
+    std::mutex ivf_mu_;
+    std::unique_ptr<IvfIndex> ivf_;
+    size_t ivf_nlist_ = 0, ivf_nprobe_ = 0, ivf_iters_ = 0;
+    DeviceBuffer d_q_, d_raw_, d_idx_, d_sim_, d_flags_, d_len_;
     PinnedBuffer h_q_, h_idx_, h_sim_;
     std::unordered_map<uint64_t, std::vector<Neighbor>> cache_single_;
     std::unordered_map<uint64_t, std::vector<std::vector<Neighbor>>> cache_batch_;

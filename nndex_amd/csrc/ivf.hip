@@ -1,0 +1,715 @@
+// IVF (inverted file) prefilter on the GPU: spherical k-means build, cluster-
+// contiguous reorder, probe and exact scan of the probed lists.
+//
+// Reference: src/approx.rs (ApproxIndex::build :46-100, kmeans_plusplus_init
+// :357-406, pick_medoid_row :409-429, spherical_kmeans :434-535,
+// candidate_indices :110-138) as the reference GPU backend uses it
+// (src/gpu.rs:435-462: built lazily from the padded rows, probed with the
+// simsimd dot, candidates rescored with the same dot into a top-k).
+//
+// Build, on the device:
+//   * medoid: f64 column means, then the row with the largest dot with the
+//     mean (strict >, the lowest row wins ties);
+//   * k-means++ seeding with the reference's rule and generator: distance
+//     max(0, 1 - dot) to the newest centroid, running minimum per row, f64
+//     totals, LCG state = state * 6364136223846793005 + 1 seeded with
+//     rows * 2654435761 + dims, first row whose cumulative distance reaches
+//     state / u64::MAX * total, duplicate of the last centroid when total <= 0;
+//   * Lloyd iterations: assignment = FP32 MFMA GEMM of rows x centroids with
+//     each row's two best centroids; a row whose two best approximate scores
+//     lie within the rounding bound is re-evaluated in the reference dot
+//     order, so the argmax equals the reference's (lowest id on ties);
+//     update = stable radix sort of (cluster, row), then one block per cluster
+//     sums its rows in row order in f64, takes the mean and normalises
+//     (empty cluster -> zero centroid);
+//   * reorder: rows in cluster order (row order inside a cluster) into a
+//     second tiled FP32 copy plus the reordered -> original map.
+// Search: centroid scores and the top nprobe clusters through the exact path
+// of a centroid index, then each probed list scanned in the reference dot
+// order into a per-list top-k, merged per query.  Every score is the exact
+// FP32 value the reference computes.
+#include <cstring>  // rocPRIM's headers use memset
+
+#include <chrono>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <vector>
+
+#include "batch.hpp"
+#include "common.hpp"
+#include "gpu_index.hpp"
+#include "ivf.hpp"
+#include "wave_topk.hpp"
+
+namespace nndex {
+
+// Reference dot of a tiled row with a row-major vector, in the simsimd
+// four-lane order (src/approx.rs:538-551 dot_small for len >= 8).
+__device__ inline float tiled_dot(const float4 *__restrict__ x, uint32_t row, uint32_t d4,
+                                  const float4 *__restrict__ v) {
+    const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
+    float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+    for (uint32_t c = 0; c < d4; ++c) {
+        const float4 a = b[(size_t)c * 64], q = v[c];
+        l0 = __builtin_fmaf(a.x, q.x, l0);
+        l1 = __builtin_fmaf(a.y, q.y, l1);
+        l2 = __builtin_fmaf(a.z, q.z, l2);
+        l3 = __builtin_fmaf(a.w, q.w, l3);
+    }
+    return (l0 + l1) + (l2 + l3);
+}
+
+// Key for an argmax under IEEE `>` with the first (lowest) id kept on ties:
+// -0.0 and +0.0 compare equal there, so both map to +0.0 (x + 0.0f).
+__device__ inline uint64_t argmax_key(float s, uint32_t id) { return make_key(s + 0.0f, id); }
+
+// Maximum over the block (every thread receives it); red has a slot per wave.
+__device__ inline uint64_t block_max_u64(uint64_t v, uint64_t *red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    const uint32_t wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (__lane_id() == 0) red[wid] = v;
+    __syncthreads();
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < nw; ++i) m = red[i] > m ? red[i] : m;
+    return m;
+}
+
+// ---- medoid (pick_medoid_row, approx.rs:409-429) ---------------------------
+
+// Per-block f64 column sums over a contiguous row range: partial[b][dim].
+__global__ __launch_bounds__(256) void colsum_kernel(const float4 *__restrict__ x, uint32_t rows,
+                                                     uint32_t d4, uint32_t rows_per_block,
+                                                     double *__restrict__ partial) {
+    const uint32_t c = threadIdx.x;  // float4 chunk
+    if (c >= d4) return;
+    const uint32_t r0 = blockIdx.x * rows_per_block;
+    const uint32_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll 4
+    for (uint32_t r = r0; r < r1; ++r) {
+        const float4 v = x[((size_t)(r / 64) * d4 + c) * 64 + r % 64];
+        s0 += v.x;
+        s1 += v.y;
+        s2 += v.z;
+        s3 += v.w;
+    }
+    double *p = partial + (size_t)blockIdx.x * d4 * 4 + c * 4;
+    p[0] = s0;
+    p[1] = s1;
+    p[2] = s2;
+    p[3] = s3;
+}
+
+__global__ __launch_bounds__(256) void colmean_kernel(const double *__restrict__ partial,
+                                                      uint32_t nblocks, uint32_t pdims,
+                                                      uint32_t rows, float *__restrict__ mean) {
+    for (uint32_t d = threadIdx.x; d < pdims; d += 256) {
+        double s = 0;
+        for (uint32_t b = 0; b < nblocks; ++b) s += partial[(size_t)b * pdims + d];
+        mean[d] = (float)(s * (1.0 / (double)rows));
+    }
+}
+
+// Row argmax of the reference dot with v (lowest row on ties).
+__global__ __launch_bounds__(256) void dot_argmax_kernel(const float4 *__restrict__ x,
+                                                         uint32_t rows, uint32_t d4,
+                                                         const float4 *__restrict__ v,
+                                                         unsigned long long *__restrict__ best) {
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long key = 0;
+    if (row < rows) key = argmax_key(tiled_dot(x, row, d4, v), row);
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(key, o);
+        key = other > key ? other : key;
+    }
+    if (__lane_id() == 0) atomicMax(best, key);
+}
+
+__device__ inline void copy_row(const float4 *__restrict__ x, uint32_t row, uint32_t d4,
+                                float4 *__restrict__ dst) {
+    for (uint32_t c = threadIdx.x; c < d4; c += blockDim.x)
+        dst[c] = x[((size_t)(row / 64) * d4 + c) * 64 + row % 64];
+}
+
+__global__ void copy_key_row_kernel(const float4 *__restrict__ x, uint32_t d4,
+                                    const unsigned long long *__restrict__ key,
+                                    float4 *__restrict__ dst) {
+    copy_row(x, key_row(*key), d4, dst);
+}
+
+// ---- k-means++ seeding (kmeans_plusplus_init, approx.rs:357-406) ------------
+
+// min_dist[r] = min(min_dist[r], max(0, 1 - dot(row, last))); f64 block sums.
+__global__ __launch_bounds__(256) void kpp_update_kernel(const float4 *__restrict__ x,
+                                                         uint32_t rows, uint32_t d4,
+                                                         const float4 *__restrict__ last,
+                                                         float *__restrict__ min_dist,
+                                                         double *__restrict__ partial) {
+    __shared__ double red[256];
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    double md = 0.0;
+    if (row < rows) {
+        float d = 1.0f - tiled_dot(x, row, d4, last);
+        if (d < 0.0f) d = 0.0f;
+        float m = min_dist[row];
+        if (d < m) {
+            m = d;
+            min_dist[row] = m;
+        }
+        md = (double)m;
+    }
+    red[threadIdx.x] = md;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// One block: total, generator step, then the first row whose cumulative
+// distance reaches the target, found top-down (ranges of blocks, blocks,
+// rows) with each sequential walk over LDS.  Writes centroid c.
+__global__ __launch_bounds__(256) void kpp_choose_kernel(const float4 *__restrict__ x,
+                                                         uint32_t rows, uint32_t d4,
+                                                         const float *__restrict__ min_dist,
+                                                         const double *__restrict__ partial,
+                                                         uint32_t nblocks,
+                                                         unsigned long long *__restrict__ rng,
+                                                         float4 *__restrict__ centroids,
+                                                         uint32_t c) {
+    __shared__ double buf[256];
+    __shared__ double base, target;
+    __shared__ int sel, dup, found;
+    __shared__ uint32_t blk, chosen;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nblocks + 255) / 256;
+    {
+        double s = 0;
+        const uint32_t b0 = t * per, b1 = (t + 1) * per < nblocks ? (t + 1) * per : nblocks;
+        for (uint32_t b = b0; b < b1; ++b) s += partial[b];
+        buf[t] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double total = 0;
+        for (int i = 0; i < 256; ++i) total += buf[i];
+        dup = !(total > 0.0);
+        sel = -1;
+        base = 0;
+        if (!dup) {
+            const unsigned long long st = *rng * 6364136223846793005ull + 1ull;
+            *rng = st;
+            target = ((double)st / (double)0xFFFFFFFFFFFFFFFFull) * total;
+            double cum = 0;
+            for (int i = 0; i < 256; ++i) {
+                if (cum + buf[i] >= target) {
+                    sel = i;
+                    break;
+                }
+                cum += buf[i];
+            }
+            base = cum;
+        }
+        chosen = rows - 1;
+    }
+    __syncthreads();
+    float4 *dst = centroids + (size_t)c * d4;
+    if (dup) {
+        for (uint32_t k = t; k < d4; k += blockDim.x) dst[k] = centroids[(size_t)(c - 1) * d4 + k];
+        return;
+    }
+    if (sel >= 0) {
+        // the block holding the crossing, 256 block sums at a time
+        const uint32_t b0 = sel * per, b1 = b0 + per < nblocks ? b0 + per : nblocks;
+        if (t == 0) blk = b1 - 1;  // rounding fallback: the range's last block
+        for (uint32_t j0 = b0; j0 < b1; j0 += 256) {
+            const uint32_t n = b1 - j0 < 256 ? b1 - j0 : 256;
+            __syncthreads();
+            if (t < n) buf[t] = partial[j0 + t];
+            __syncthreads();
+            if (t == 0) {
+                found = 0;
+                double cum = base;
+                for (uint32_t i = 0; i < n; ++i) {
+                    if (cum + buf[i] >= target) {
+                        blk = j0 + i;
+                        found = 1;
+                        break;
+                    }
+                    cum += buf[i];
+                }
+                base = cum;
+            }
+            __syncthreads();
+            if (found) break;
+        }
+        __syncthreads();
+        // the row inside that block
+        const uint32_t r0 = blk * 256, n = rows - r0 < 256 ? rows - r0 : 256;
+        if (t < n) buf[t] = (double)min_dist[r0 + t];
+        __syncthreads();
+        if (t == 0) {
+            double cum = base;
+            uint32_t pick = r0 + n - 1;
+            for (uint32_t i = 0; i < n; ++i) {
+                cum += buf[i];
+                if (cum >= target) {
+                    pick = r0 + i;
+                    break;
+                }
+            }
+            chosen = pick;
+        }
+        __syncthreads();
+    }
+    copy_row(x, chosen, d4, dst);
+}
+
+__global__ void fill_f32_kernel(float *p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = v;
+}
+
+__global__ void iota_kernel(uint32_t *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = i;
+}
+
+// Cluster sizes: per-block LDS histogram, then one global add per cluster.
+__global__ __launch_bounds__(256) void histogram_kernel(const uint32_t *__restrict__ assign,
+                                                        uint32_t rows, uint32_t nlist,
+                                                        uint32_t *__restrict__ counts) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t i = threadIdx.x; i < nlist; i += 256) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < rows; i += gridDim.x * 256)
+        atomicAdd(hist + assign[i], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nlist; i += 256)
+        if (hist[i]) atomicAdd(counts + i, hist[i]);
+}
+
+// Exclusive scan of nlist counts into offsets[nlist + 1] (one block).
+__global__ __launch_bounds__(256) void offsets_kernel(const uint32_t *__restrict__ counts,
+                                                      uint32_t nlist,
+                                                      uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t part[256];
+    const uint32_t per = (nlist + 255) / 256, t = threadIdx.x;
+    const uint32_t i0 = t * per, i1 = (t + 1) * per < nlist ? (t + 1) * per : nlist;
+    uint32_t s = 0;
+    for (uint32_t i = i0; i < i1; ++i) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint32_t v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    uint32_t acc = part[t];
+    for (uint32_t i = i0; i < i1; ++i) {
+        offsets[i] = acc;
+        acc += counts[i];
+    }
+    if (t == 255) offsets[nlist] = acc;
+}
+
+// Centroid update (spherical_kmeans, approx.rs:490-531): one block per
+// cluster; thread c owns float4 chunk c and sums the cluster's rows in row
+// order in f64; the squared norm of the mean is summed in dimension order.
+__global__ __launch_bounds__(256) void centroid_update_kernel(const float4 *__restrict__ x,
+                                                              uint32_t d4,
+                                                              const uint32_t *__restrict__ sorted_rows,
+                                                              const uint32_t *__restrict__ offsets,
+                                                              float4 *__restrict__ centroids) {
+    __shared__ double scaled[512];
+    __shared__ double nsq;
+    const uint32_t cl = blockIdx.x, t = threadIdx.x;
+    const uint32_t r0 = offsets[cl], r1 = offsets[cl + 1];
+    const uint32_t count = r1 - r0;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (t < d4) {
+#pragma unroll 4
+        for (uint32_t i = r0; i < r1; ++i) {
+            const uint32_t r = sorted_rows[i];
+            const float4 v = x[((size_t)(r / 64) * d4 + t) * 64 + r % 64];
+            s0 += v.x;
+            s1 += v.y;
+            s2 += v.z;
+            s3 += v.w;
+        }
+    }
+    const double inv_count = count ? 1.0 / (double)count : 0.0;
+    if (t < d4) {
+        scaled[4 * t + 0] = s0 * inv_count;
+        scaled[4 * t + 1] = s1 * inv_count;
+        scaled[4 * t + 2] = s2 * inv_count;
+        scaled[4 * t + 3] = s3 * inv_count;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double acc = 0;
+        for (uint32_t i = 0; i < 4 * d4; ++i) acc += scaled[i] * scaled[i];
+        nsq = acc;
+    }
+    __syncthreads();
+    if (t < d4) {
+        float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (count) {
+            const double inv_norm = nsq > 1e-30 ? 1.0 / sqrt(nsq) : 1.0;
+            out = make_float4((float)(s0 * inv_count * inv_norm), (float)(s1 * inv_count * inv_norm),
+                              (float)(s2 * inv_count * inv_norm),
+                              (float)(s3 * inv_count * inv_norm));
+        }
+        centroids[(size_t)cl * d4 + t] = out;
+    }
+}
+
+// Exact argmax for the rows the MFMA pass could not separate: the reference
+// dot (four-lane order) of the row with every centroid, lowest id on ties.
+__global__ __launch_bounds__(256) void recheck_kernel(const float4 *__restrict__ x, uint32_t d4,
+                                                      const float4 *__restrict__ centroids,
+                                                      uint32_t nlist,
+                                                      const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ count,
+                                                      uint32_t *__restrict__ assign) {
+    __shared__ float4 srow[128];
+    __shared__ uint64_t red[4];
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t row = list[i];
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < d4; c += 256)
+            srow[c] = x[((size_t)(row / 64) * d4 + c) * 64 + row % 64];
+        __syncthreads();
+        uint64_t best = 0;
+        for (uint32_t cl = threadIdx.x; cl < nlist; cl += 256) {
+            const float4 *cv = centroids + (size_t)cl * d4;
+            float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+            for (uint32_t c = 0; c < d4; ++c) {
+                const float4 a = srow[c], b = cv[c];
+                l0 = __builtin_fmaf(a.x, b.x, l0);
+                l1 = __builtin_fmaf(a.y, b.y, l1);
+                l2 = __builtin_fmaf(a.z, b.z, l2);
+                l3 = __builtin_fmaf(a.w, b.w, l3);
+            }
+            const uint64_t key = argmax_key((l0 + l1) + (l2 + l3), cl);
+            best = key > best ? key : best;
+        }
+        best = block_max_u64(best, red);
+        if (threadIdx.x == 0) assign[row] = key_row(best);
+    }
+}
+
+// Reorder: reordered row j = row sorted[j] (both tiled, padding rows zero).
+__global__ __launch_bounds__(256) void gather_sorted_kernel(const float4 *__restrict__ x,
+                                                            uint32_t rows, uint32_t d4,
+                                                            const uint32_t *__restrict__ sorted,
+                                                            float4 *__restrict__ y) {
+    const size_t total = (size_t)((rows + 63) / 64) * 64 * d4;
+    for (size_t u = (size_t)blockIdx.x * 256 + threadIdx.x; u < total;
+         u += (size_t)gridDim.x * 256) {
+        const size_t tile = u / ((size_t)d4 * 64), rem = u % ((size_t)d4 * 64);
+        const uint32_t c = rem / 64, r = rem % 64;
+        const size_t j = tile * 64 + r;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < rows) {
+            const uint32_t src = sorted[j];
+            v = x[((size_t)(src / 64) * d4 + c) * 64 + src % 64];
+        }
+        y[u] = v;
+    }
+}
+
+// ---- search -----------------------------------------------------------------
+
+// Block (query q, probe p): the list's rows of the reordered copy scored in
+// the reference dot order; the best k keys (original row ids) per list.
+template <int CW>
+__global__ __launch_bounds__(256) void ivf_scan_kernel(IvfScanArgs a) {
+    __shared__ uint64_t slots[4][CW];
+    __shared__ uint32_t cnt[4];
+    const uint32_t q = blockIdx.x / a.nprobe, p = blockIdx.x % a.nprobe;
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
+    const uint32_t cl = (uint32_t)a.probe[(size_t)q * a.probe_stride + p];
+    const uint32_t r0 = a.offsets[cl], r1 = a.offsets[cl + 1];
+    const_as<f32x4> *Q = as_constant(reinterpret_cast<const f32x4 *>(a.q + (size_t)q * a.d4 * 4));
+    WaveTopK tk;
+    tk.init(slots[wid], CW, a.k, nullptr);
+    for (uint32_t t0 = r0 / 64 + wid; t0 * 64 < r1; t0 += 4) {
+        const uint32_t j = t0 * 64 + lane;
+        const bool valid = j >= r0 && j < r1;
+        const float4 *b = a.x + (size_t)t0 * a.d4 * 64 + lane;
+        float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+        for (uint32_t c = 0; c < a.d4; ++c) {
+            const float4 v = b[(size_t)c * 64];
+            const f32x4 qv = Q[c];
+            l0 = __builtin_fmaf(v.x, qv.x, l0);
+            l1 = __builtin_fmaf(v.y, qv.y, l1);
+            l2 = __builtin_fmaf(v.z, qv.z, l2);
+            l3 = __builtin_fmaf(v.w, qv.w, l3);
+        }
+        const float s = (l0 + l1) + (l2 + l3);
+        tk.offer(valid ? make_key(s, a.orig[j]) : 0, valid);
+    }
+    tk.compact();
+    if (lane == 0) cnt[wid] = tk.count;
+    __syncthreads();
+    if (wid != 0) return;
+    for (uint32_t w = 1; w < 4; ++w)
+        for (uint32_t i0 = 0; i0 < cnt[w]; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            tk.offer(i < cnt[w] ? slots[w][i] : 0, i < cnt[w]);
+        }
+    tk.compact();
+    uint64_t *dst = a.part + ((size_t)q * a.nprobe + p) * a.k;
+    for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk.count ? slots[0][i] : 0;
+}
+
+// One block per query: its nprobe lists merged into the final top-k.
+template <int CW>
+__global__ __launch_bounds__(256) void ivf_merge_kernel(IvfScanArgs a) {
+    __shared__ uint64_t slots[4][CW];
+    __shared__ uint32_t cnt[4];
+    const uint32_t q = blockIdx.x, lane = __lane_id(), wid = threadIdx.x >> 6;
+    WaveTopK tk;
+    tk.init(slots[wid], CW, a.k, nullptr);
+    const uint64_t *src = a.part + (size_t)q * a.nprobe * a.k;
+    const uint32_t n = a.nprobe * a.k;
+    for (uint32_t i0 = wid * 64; i0 < n; i0 += 256) {
+        const uint32_t i = i0 + lane;
+        const uint64_t key = i < n ? src[i] : 0;
+        tk.offer(key, key != 0);
+    }
+    tk.compact();
+    if (lane == 0) cnt[wid] = tk.count;
+    __syncthreads();
+    if (wid != 0) return;
+    for (uint32_t w = 1; w < 4; ++w)
+        for (uint32_t i0 = 0; i0 < cnt[w]; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            tk.offer(i < cnt[w] ? slots[w][i] : 0, i < cnt[w]);
+        }
+    tk.compact();
+    for (uint32_t i = lane; i < a.k; i += 64) {
+        const uint64_t key = i < tk.count ? slots[0][i] : 0;
+        a.out_idx[(size_t)q * a.k + i] = key ? key_row(key) : 0;
+        a.out_sim[(size_t)q * a.k + i] = key ? key_sim(key) : 0.f;
+    }
+    if (lane == 0) a.out_len[q] = tk.count;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+size_t ivf_isqrt(size_t n) {
+    if (n == 0) return 0;
+    size_t x = n, y = (x + 1) / 2;
+    while (y < x) {
+        x = y;
+        y = (x + n / x) / 2;
+    }
+    return x;
+}
+
+size_t ivf_choose_nlist(size_t rows) {
+    if (rows < kMinRowsForIvf) return 1;
+    const size_t s = ivf_isqrt(rows);
+    return s < 16 ? 16 : (s > 512 ? 512 : s);
+}
+
+size_t ivf_choose_nprobe(size_t nlist, size_t k, size_t rows) {
+    const size_t pct = k >= 50 ? 8 : (k >= 20 ? 6 : 4);
+    size_t nprobe = (nlist * pct + 99) / 100;
+    if (nprobe < 1) nprobe = 1;
+    size_t avg = rows / (nlist ? nlist : 1);
+    if (avg < 1) avg = 1;
+    size_t min_np = (k * 40 + avg - 1) / avg;
+    if (min_np < 1) min_np = 1;
+    if (nprobe < min_np) nprobe = min_np;
+    if (nprobe > nlist) nprobe = nlist;
+    return nprobe < 1 ? 1 : nprobe;
+}
+
+IvfIndex::~IvfIndex() {
+    for (void *p : {(void *)centroids, (void *)offsets, (void *)orig, (void *)assign, (void *)xr})
+        if (p) (void)hipFree(p);
+}
+
+static uint32_t ivf_cw(uint32_t k) {
+    uint32_t cw = 128;
+    while (cw < k + 64) cw <<= 1;
+    return cw;
+}
+
+void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uint32_t nlist,
+               uint32_t iters, double x_max, uint32_t ncu, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t d4 = pdims / 4;
+    if (pdims > 512)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF build supports padded dims <= 512");
+    if (nlist < 2 || nlist > rows || nlist > 16384)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "nlist must be in [2, min(rows, 16384)]");
+    ivf.rows = rows;
+    ivf.pdims = pdims;
+    ivf.nlist = nlist;
+    ivf.iters = iters;
+    const size_t prow = padded_rows(rows);
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.centroids, (size_t)nlist * pdims * sizeof(float)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.offsets, (nlist + 1) * sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.orig, (size_t)rows * sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.assign, (size_t)rows * sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.xr, prow * d4 * sizeof(float4)));
+
+    const uint32_t nb = (rows + 255) / 256;
+    const uint32_t cs_rows = 4096, cs_blocks = (rows + cs_rows - 1) / cs_rows;
+    double *partial = nullptr, *colpart = nullptr;
+    float *min_dist = nullptr, *mean = nullptr;
+    unsigned long long *rng = nullptr, *medoid = nullptr;
+    uint32_t *sorted_keys = nullptr, *rows_iota = nullptr, *counts = nullptr, *recheck = nullptr,
+             *recheck_count = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    auto free_all = [&] {
+        for (void *p : {(void *)partial, (void *)colpart, (void *)min_dist, (void *)mean,
+                        (void *)rng, (void *)medoid, (void *)sorted_keys, (void *)rows_iota,
+                        (void *)counts, (void *)recheck, (void *)recheck_count, sort_tmp})
+            if (p) (void)hipFree(p);
+    };
+    try {
+        NNDEX_HIP_CHECK(hipMalloc(&partial, nb * sizeof(double)));
+        NNDEX_HIP_CHECK(hipMalloc(&colpart, (size_t)cs_blocks * pdims * sizeof(double)));
+        NNDEX_HIP_CHECK(hipMalloc(&min_dist, (size_t)rows * sizeof(float)));
+        NNDEX_HIP_CHECK(hipMalloc(&mean, pdims * sizeof(float)));
+        NNDEX_HIP_CHECK(hipMalloc(&rng, sizeof(unsigned long long)));
+        NNDEX_HIP_CHECK(hipMalloc(&medoid, sizeof(unsigned long long)));
+        NNDEX_HIP_CHECK(hipMalloc(&sorted_keys, (size_t)rows * sizeof(uint32_t)));
+        NNDEX_HIP_CHECK(hipMalloc(&rows_iota, (size_t)rows * sizeof(uint32_t)));
+        NNDEX_HIP_CHECK(hipMalloc(&counts, nlist * sizeof(uint32_t)));
+        NNDEX_HIP_CHECK(hipMalloc(&recheck, (size_t)rows * sizeof(uint32_t)));
+        NNDEX_HIP_CHECK(hipMalloc(&recheck_count, sizeof(uint32_t)));
+        uint32_t bits = 1;
+        while ((1u << bits) < nlist) ++bits;
+        NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ivf.assign, sorted_keys,
+                                                  rows_iota, ivf.orig, rows, 0, bits, s));
+        NNDEX_HIP_CHECK(hipMalloc(&sort_tmp, sort_bytes));
+
+        float4 *cent = reinterpret_cast<float4 *>(ivf.centroids);
+        // first centroid: the medoid row
+        hipLaunchKernelGGL(colsum_kernel, dim3(cs_blocks), dim3(256), 0, s, x, rows, d4, cs_rows,
+                           colpart);
+        hipLaunchKernelGGL(colmean_kernel, dim3(1), dim3(256), 0, s, colpart, cs_blocks, pdims,
+                           rows, mean);
+        NNDEX_HIP_CHECK(hipMemsetAsync(medoid, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(dot_argmax_kernel, dim3(nb), dim3(256), 0, s, x, rows, d4,
+                           reinterpret_cast<const float4 *>(mean), medoid);
+        hipLaunchKernelGGL(copy_key_row_kernel, dim3(1), dim3(256), 0, s, x, d4, medoid, cent);
+        // k-means++ seeding
+        const unsigned long long seed = (unsigned long long)rows * 2654435761ull + pdims;
+        NNDEX_HIP_CHECK(hipMemcpyAsync(rng, &seed, sizeof(seed), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(1024), dim3(256), 0, s, min_dist, (size_t)rows,
+                           3.40282347e38f);
+        for (uint32_t c = 1; c < nlist; ++c) {
+            hipLaunchKernelGGL(kpp_update_kernel, dim3(nb), dim3(256), 0, s, x, rows, d4,
+                               cent + (size_t)(c - 1) * d4, min_dist, partial);
+            hipLaunchKernelGGL(kpp_choose_kernel, dim3(1), dim3(256), 0, s, x, rows, d4, min_dist,
+                               partial, nb, rng, cent, c);
+        }
+        NNDEX_HIP_CHECK(hipGetLastError());
+
+        // Lloyd iterations.  The MFMA pass and the reference dot each lie
+        // within gamma_n |x| |c| of the exact dot (|c| <= max(1, |x|max)), so
+        // a top-two gap above four such bounds fixes the reference's argmax;
+        // the margin used is twice that.
+        const double u = 0x1p-24, n = pdims;
+        const double gamma = n * u / (1.0 - n * u);
+        const double cmax = x_max > 1.0 ? x_max : 1.0;
+        const float tol = (float)(8.0 * gamma * x_max * cmax * (1.0 + 1e-6));
+        hipLaunchKernelGGL(iota_kernel, dim3(1024), dim3(256), 0, s, rows_iota, rows);
+        const size_t hist_lds = (size_t)nlist * sizeof(uint32_t);
+        std::vector<uint32_t> rechecked(iters, 0);
+        for (uint32_t it = 0; it < iters; ++it) {
+            launch_zero_u32(recheck_count, 1, s);
+            MfmaScanArgs m{};
+            m.x = reinterpret_cast<const uint4 *>(x);
+            m.rows = rows;
+            m.rows_padded = static_cast<uint32_t>(prow);
+            m.dch = d4;
+            m.q = reinterpret_cast<const uint4 *>(ivf.centroids);
+            m.q_stride = d4;
+            m.nq = nlist;
+            m.mode = 2;
+            m.assign = ivf.assign;
+            m.tol = tol;
+            m.recheck = recheck;
+            m.recheck_count = recheck_count;
+            const uint32_t br = mfma_block_rows(false, m.dch);
+            uint32_t grid = (m.rows_padded + br - 1) / br;
+            if (grid > ncu) grid = ncu;
+            launch_mfma_scan(m, false, grid, s);
+            hipLaunchKernelGGL(recheck_kernel, dim3(ncu * 4), dim3(256), 0, s, x, d4,
+                               reinterpret_cast<const float4 *>(ivf.centroids), nlist, recheck,
+                               recheck_count, ivf.assign);
+            NNDEX_HIP_CHECK(hipMemcpyAsync(&rechecked[it], recheck_count, sizeof(uint32_t),
+                                           hipMemcpyDeviceToHost, s));
+            launch_zero_u32(counts, nlist, s);
+            hipLaunchKernelGGL(histogram_kernel, dim3(256), dim3(256), hist_lds, s, ivf.assign,
+                               rows, nlist, counts);
+            hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(256), 0, s, counts, nlist,
+                               ivf.offsets);
+            NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ivf.assign,
+                                                      sorted_keys, rows_iota, ivf.orig, rows, 0,
+                                                      bits, s));
+            const uint32_t ublock = (d4 + 63) / 64 * 64;
+            hipLaunchKernelGGL(centroid_update_kernel, dim3(nlist), dim3(ublock), 0, s, x, d4,
+                               ivf.orig, ivf.offsets, cent);
+            NNDEX_HIP_CHECK(hipGetLastError());
+        }
+        // reorder by the last assignment (approx.rs:64-89); orig and offsets
+        // already hold the stable cluster order of that assignment
+        const size_t total = prow * d4;
+        uint32_t grid = (uint32_t)((total + 255) / 256);
+        if (grid > 65535u * 8) grid = 65535u * 8;
+        hipLaunchKernelGGL(gather_sorted_kernel, dim3(grid), dim3(256), 0, s, x, rows, d4,
+                           ivf.orig, ivf.xr);
+        NNDEX_HIP_CHECK(hipGetLastError());
+        NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+        ivf.rechecked = 0;
+        for (uint32_t r : rechecked) ivf.rechecked += r;
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        free_all();
+        throw;
+    }
+    free_all();
+    ivf.build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void ivf_scan(const IvfScanArgs &a, uint32_t nq, hipStream_t s) {
+    if (nq == 0) return;
+#define NNDEX_IVF_CASE(C)                                                                \
+    case C:                                                                              \
+        hipLaunchKernelGGL(ivf_scan_kernel<C>, dim3(nq * a.nprobe), dim3(256), 0, s, a); \
+        hipLaunchKernelGGL(ivf_merge_kernel<C>, dim3(nq), dim3(256), 0, s, a);           \
+        break;
+    switch (ivf_cw(a.k)) {
+        NNDEX_IVF_CASE(128)
+        NNDEX_IVF_CASE(256)
+        NNDEX_IVF_CASE(512)
+        NNDEX_IVF_CASE(1024)
+    default:
+        throw Error(NNDEX_ERR_UNSUPPORTED, "IVF search supports k <= 960");
+    }
+#undef NNDEX_IVF_CASE
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace nndex

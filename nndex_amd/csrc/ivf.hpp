@@ -1,0 +1,61 @@
+// IVF prefilter on the device (ivf.hip).  Mirrors ApproxIndex of
+// src/approx.rs:11-25: centroids, a cluster-contiguous reordered copy of the
+// rows, the reordered -> original row map and the per-cluster offsets.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+
+namespace nndex {
+
+class GpuIndex;
+
+// approx.rs constants and heuristics (MIN_ROWS_FOR_IVF :31, KMEANS_MAX_ITERS
+// :28, choose_nlist :305-311, choose_nprobe :318-336, isqrt :339-351).
+constexpr size_t kMinRowsForIvf = 256;
+constexpr uint32_t kKmeansMaxIters = 12;
+size_t ivf_isqrt(size_t n);
+size_t ivf_choose_nlist(size_t rows);
+size_t ivf_choose_nprobe(size_t nlist, size_t k, size_t rows);
+
+struct IvfIndex {
+    uint32_t rows = 0, pdims = 0, nlist = 0, iters = 0;
+    float *centroids = nullptr;   // [nlist][pdims] row-major
+    uint32_t *offsets = nullptr;  // [nlist + 1]
+    uint32_t *orig = nullptr;     // [rows] original row of each reordered position
+    uint32_t *assign = nullptr;   // [rows] final cluster of each original row
+    float4 *xr = nullptr;         // reordered rows, tiled FP32
+    uint32_t rechecked = 0;       // rows re-evaluated exactly, summed over iterations
+    double build_ms = 0.0;
+    std::unique_ptr<GpuIndex> cindex;  // centroid index used for probing
+    // ApproxIndex::provides_speedup (approx.rs:34-36)
+    bool provides_speedup() const { return rows >= kMinRowsForIvf && nlist >= 4; }
+    ~IvfIndex();
+};
+
+struct IvfScanArgs {
+    const float4 *x;            // reordered rows, tiled
+    uint32_t d4;
+    const float *q;             // [nq][d4 * 4] normalised padded queries
+    const uint64_t *probe;      // [nq][probe_stride] probed cluster ids
+    uint32_t probe_stride;
+    uint32_t nprobe;
+    const uint32_t *offsets;
+    const uint32_t *orig;
+    uint32_t k;
+    uint64_t *part;             // [nq][nprobe][k] per-list best keys
+    uint64_t *out_idx;          // [nq][k]
+    float *out_sim;             // [nq][k]
+    uint64_t *out_len;          // [nq]
+};
+
+// Build the index of `rows` tiled rows (pdims padded columns) on stream s.
+// x_max is the largest row norm (bounds the assignment's rounding error).
+void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uint32_t nlist,
+               uint32_t iters, double x_max, uint32_t ncu, hipStream_t s);
+// Scan the probed lists and merge per query (probe ids already on device).
+void ivf_scan(const IvfScanArgs &a, uint32_t nq, hipStream_t s);
+
+}  // namespace nndex

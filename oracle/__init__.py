@@ -100,6 +100,7 @@ def lib() -> ctypes.CDLL:
             "orc_ivf_assignments": (_U64, [_P]),
             "orc_ivf_search": (_SZ, [_P, _F, _SZ, _SZ, _U64, _F]),
             "orc_ivf_probe": (_SZ, [_P, _F, _SZ, _U64]),
+            "orc_ivf_search_gpu": (_SZ, [_P, _F, _F, _SZ, _SZ, _U64, _F]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -359,6 +360,16 @@ class Ivf:
         idx = np.zeros(kk, np.uint64)
         sim = np.zeros(kk, np.float32)
         got = lib().orc_ivf_search(self._h, _fp(q), k, nprobe, idx.ctypes.data_as(_U64), _fp(sim))
+        return idx[:got], sim[:got]
+
+    def search_gpu(self, query, k: int, nprobe: int = 0):
+        """The GPU backend's approximate search (src/gpu.rs:448-462)."""
+        q = _f32(query).ravel()
+        kk = max(min(k, self.rows), 1)
+        idx = np.zeros(kk, np.uint64)
+        sim = np.zeros(kk, np.float32)
+        got = lib().orc_ivf_search_gpu(self._h, _fp(self._m), _fp(q), k, nprobe,
+                                       idx.ctypes.data_as(_U64), _fp(sim))
         return idx[:got], sim[:got]
 
     def probe(self, query, nprobe: int) -> np.ndarray:

@@ -925,10 +925,15 @@ const uint64_t *orc_ivf_assignments(const orc_ivf *ivf) { return ivf->assignment
 /* candidate_clusters, approx.rs:159-215: centroid scores (ndarray GEMV when
  * nlist>=16 && dims>=16, else dot_small), top nprobe by score desc.
  * select_nth_unstable_by has no index tie-break; ties resolve to lower id here. */
-size_t orc_ivf_probe(const orc_ivf *ivf, const float *query, size_t nprobe, uint64_t *clusters) {
+/* Top-nprobe clusters by centroid score (score desc, id asc).  gemv != 0
+ * scores with ndarray's dot (candidate_clusters, approx.rs:158-196, used by
+ * the CPU backend); gemv == 0 with dot_small (candidate_indices,
+ * approx.rs:110-138, used by the GPU backend). */
+static size_t ivf_probe_impl(const orc_ivf *ivf, const float *query, size_t nprobe,
+                             uint64_t *clusters, int allow_gemv) {
     size_t nl = ivf->nlist;
     item_t *sc = (item_t *)malloc(nl * sizeof(item_t));
-    int gemv = nl >= 16 && ivf->dims >= 16;
+    int gemv = allow_gemv && nl >= 16 && ivf->dims >= 16;
     for (size_t c = 0; c < nl; ++c) {
         const float *cen = ivf->centroids + c * ivf->dims;
         sc[c].index = c;
@@ -940,6 +945,34 @@ size_t orc_ivf_probe(const orc_ivf *ivf, const float *query, size_t nprobe, uint
     for (size_t i = 0; i < nprobe; ++i) clusters[i] = sc[i].index;
     free(sc);
     return nprobe;
+}
+
+size_t orc_ivf_probe(const orc_ivf *ivf, const float *query, size_t nprobe, uint64_t *clusters) {
+    return ivf_probe_impl(ivf, query, nprobe, clusters, 1);
+}
+
+/* search_approx_cpu of the GPU backend, src/gpu.rs:448-462: candidate_indices
+ * (approx.rs:110-138: dot_small centroid scores, top nprobe), then every
+ * candidate row of the original matrix rescored with cpu_dot
+ * (src/gpu.rs:887-898, the simsimd dot for len >= 8) into a TopKAccumulator. */
+size_t orc_ivf_search_gpu(const orc_ivf *ivf, const float *matrix, const float *query, size_t k,
+                          size_t nprobe, uint64_t *idx, float *sim) {
+    size_t dims = ivf->dims;
+    size_t capped = k < ivf->rows ? k : ivf->rows;
+    orc_topk *acc = orc_topk_new(capped);
+    if (nprobe == 0) nprobe = orc_choose_nprobe(ivf->nlist, capped, ivf->rows);
+    uint64_t *cl = (uint64_t *)malloc((ivf->nlist ? ivf->nlist : 1) * sizeof(uint64_t));
+    size_t np = ivf->nlist <= 1 ? 1 : ivf_probe_impl(ivf, query, nprobe, cl, 0);
+    if (ivf->nlist <= 1) cl[0] = 0;
+    for (size_t p = 0; p < np; ++p) {
+        size_t s = ivf->offsets[cl[p]], e = ivf->offsets[cl[p] + 1];
+        for (size_t r = s; r < e; ++r) {
+            uint64_t row = ivf->original_indices[r];
+            orc_topk_push(acc, row, orc_dot_product(matrix + row * dims, query, dims));
+        }
+    }
+    free(cl);
+    return orc_topk_into_sorted(acc, idx, sim);
 }
 
 /* search_approx, cpu.rs:425-453 */

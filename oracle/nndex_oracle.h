@@ -122,6 +122,10 @@ size_t orc_ivf_search(const orc_ivf *ivf, const float *query, size_t k, size_t n
                       uint64_t *idx, float *sim);
 /* probed cluster ids of one query, sorted by (score desc, id asc); returns count */
 size_t orc_ivf_probe(const orc_ivf *ivf, const float *query, size_t nprobe, uint64_t *clusters);
+/* GPU-backend approximate search (src/gpu.rs:448-462) over `matrix`, the rows
+ * the index was built from; nprobe == 0 -> choose_nprobe(nlist, k, rows) */
+size_t orc_ivf_search_gpu(const orc_ivf *ivf, const float *matrix, const float *query, size_t k,
+                          size_t nprobe, uint64_t *idx, float *sim);
 
 #ifdef __cplusplus
 }

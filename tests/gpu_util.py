@@ -58,19 +58,19 @@ class BackendIndex:
             storage.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), self.rows, self.pdims,
             int(approx), int(enable_cache), device, ctypes.byref(self.h)))
 
-    def search(self, q: np.ndarray, k: int):
+    def search(self, q: np.ndarray, k: int, approx: bool = False):
         q = np.ascontiguousarray(q, np.float32)
         kk = min(k, self.rows)
         idx = np.zeros(kk, np.uint64)
         sim = np.zeros(kk, np.float32)
         n = ctypes.c_size_t()
         check(_lib.lib().nndex_gpu_index_search(
-            self.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), kk, 0,
+            self.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), kk, int(approx),
             idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
             sim.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(n)))
         return idx[: n.value], sim[: n.value]
 
-    def search_batch(self, q: np.ndarray, k: int):
+    def search_batch(self, q: np.ndarray, k: int, approx: bool = False):
         q = np.ascontiguousarray(q, np.float32)
         nq = q.shape[0]
         kk = min(k, self.rows)
@@ -78,11 +78,33 @@ class BackendIndex:
         sim = np.zeros((nq, kk), np.float32)
         lens = np.zeros(nq, np.uintp)
         check(_lib.lib().nndex_gpu_index_search_batch(
-            self.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nq, kk, 0,
+            self.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nq, kk, int(approx),
             idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
             sim.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
             lens.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t))))
         return idx, sim, lens
+
+    # ---- IVF ----
+    def set_ivf(self, nlist=0, nprobe=0, iters=0):
+        check(_lib.lib().nndex_gpu_index_set_ivf_params(self.h, nlist, nprobe, iters))
+
+    def build_ivf(self) -> "_lib.IvfInfo":
+        info = _lib.IvfInfo()
+        check(_lib.lib().nndex_gpu_index_build_ivf(self.h, ctypes.byref(info)))
+        return info
+
+    def ivf_export(self):
+        info = self.build_ivf()
+        nl, pd = int(info.nlist), int(info.padded_dims)
+        cent = np.zeros((max(nl, 1), pd), np.float32)
+        off = np.zeros(nl + 1, np.uint64)
+        orig = np.zeros(self.rows, np.uint64)
+        assign = np.zeros(self.rows, np.uint64)
+        u64 = ctypes.POINTER(ctypes.c_uint64)
+        check(_lib.lib().nndex_gpu_index_ivf_export(
+            self.h, cent.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            off.ctypes.data_as(u64), orig.ctypes.data_as(u64), assign.ctypes.data_as(u64)))
+        return info, cent[:nl], off, orig, assign
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -99,6 +121,14 @@ class DeviceSearch:
         check(_lib.lib().nndex_workspace_new(handle, max_queries, max_k, ctypes.byref(self.ws)))
         self.stream = ctypes.c_void_p()
         check(_lib.lib().nndex_stream_create(ctypes.byref(self.stream)))
+
+    def run_ivf(self, dq: DevBuf, nq: int, k: int, nprobe: int, didx: DevBuf, dsim: DevBuf,
+                dlen: DevBuf, sync=True):
+        check(_lib.lib().nndex_gpu_index_search_ivf_device(
+            self.handle, self.ws, dq.ptr, nq, k, nprobe, didx.ptr, dsim.ptr, dlen.ptr,
+            self.stream))
+        if sync:
+            check(_lib.lib().nndex_stream_synchronize(self.stream))
 
     def run(self, dq: DevBuf, nq: int, k: int, didx: DevBuf, dsim: DevBuf, sync=True):
         check(_lib.lib().nndex_gpu_index_search_device(self.handle, self.ws, dq.ptr, nq, k,

@@ -1,0 +1,178 @@
+"""IVF prefilter on the GPU: the k-means build and the approximate search
+against the oracle's restatement of src/approx.rs and src/gpu.rs:435-462.
+
+The build reproduces the reference's k-means (assignment argmax re-checked in
+the reference dot order, f64 centroid sums in row order), so centroids,
+assignments, offsets and the permutation are compared for equality, and
+approximate results (indices, scores and counts) bit for bit."""
+
+import numpy as np
+import pytest
+
+import nndex_amd
+from nndex_amd import synth
+
+from gpu_util import BackendIndex, DevBuf, DeviceSearch, gpu_available
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the HIP path has no CPU fallback")
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def _storage(orc, rows, dims, seed, mixture=False):
+    rng = np.random.default_rng(seed)
+    if mixture:
+        centres = rng.normal(size=(max(rows // 200, 4), dims)).astype(np.float32)
+        pick = rng.integers(0, centres.shape[0], size=rows)
+        raw = centres[pick] + 0.3 * rng.normal(size=(rows, dims)).astype(np.float32)
+    else:
+        raw = rng.normal(size=(rows, dims)).astype(np.float32)
+    return orc.normalize_rows(raw, rows, dims), rng
+
+
+@pytest.mark.parametrize("rows,dims,nlist,iters", [
+    (3000, 40, 0, 0),      # reference defaults: nlist = isqrt(3000) = 54, 12 iterations
+    (5000, 100, 0, 4),     # padded dims 104
+    (4096, 128, 64, 6),
+    (2500, 8, 16, 3),
+    (6000, 32, 300, 2),    # many small clusters, some may empty
+])
+def test_ivf_build_equals_oracle(orc, rows, dims, nlist, iters):
+    _case("ivf_build", rows=rows, dims=dims, nlist=nlist, iters=iters)
+    m, _ = _storage(orc, rows, dims, rows + dims, mixture=(dims % 2 == 0))
+    b = BackendIndex(m, approx=True)
+    b.set_ivf(nlist=nlist, iters=iters)
+    info, cent, off, orig, assign = b.ivf_export()
+    ref = orc.Ivf(m, nlist=nlist, iters=iters)
+    assert int(info.nlist) == ref.nlist
+    assert int(info.iters) == (iters or 12)
+    assert info.build_ms > 0
+    np.testing.assert_array_equal(assign, ref.assignments())
+    np.testing.assert_array_equal(off, ref.offsets())
+    np.testing.assert_array_equal(orig, ref.original_indices())
+    assert np.array_equal(cent.view(np.uint32), ref.centroids().view(np.uint32))
+
+
+@pytest.mark.parametrize("rows,dims,k,nprobe", [
+    (3000, 40, 10, 0),
+    (5000, 100, 5, 0),
+    (8000, 64, 20, 3),
+    (4096, 128, 1, 1),
+    (6000, 32, 100, 0),
+])
+def test_ivf_search_equals_oracle(orc, rows, dims, k, nprobe):
+    """Approximate single and batch search through the backend boundary equal
+    the reference GPU backend's host-side IVF search bit for bit."""
+    _case("ivf_search", rows=rows, dims=dims, k=k, nprobe=nprobe)
+    m, rng = _storage(orc, rows, dims, 3 * rows + dims, mixture=True)
+    qs = orc.normalize_rows(rng.normal(size=(12, dims)).astype(np.float32), 12, dims)
+    b = BackendIndex(m, approx=True)
+    b.set_ivf(nprobe=nprobe)
+    ref = orc.Ivf(m)
+    for q in qs[:3]:
+        idx, sim = b.search(q, k, approx=True)
+        ridx, rsim = ref.search_gpu(q, k, nprobe)
+        np.testing.assert_array_equal(idx, ridx)
+        assert np.array_equal(sim.view(np.uint32), rsim.view(np.uint32))
+    bidx, bsim, lens = b.search_batch(qs, k, approx=True)
+    for i, q in enumerate(qs):
+        ridx, rsim = ref.search_gpu(q, k, nprobe)
+        assert int(lens[i]) == len(ridx)
+        np.testing.assert_array_equal(bidx[i, : lens[i]], ridx)
+        assert np.array_equal(bsim[i, : lens[i]].view(np.uint32), rsim.view(np.uint32))
+
+
+def test_ivf_device_batch_and_nprobe(orc):
+    """The device entry point with explicit nprobe over a batch above 8 queries
+    (centroid probe on the certified MFMA path); nprobe = nlist equals the
+    exact search."""
+    rows, dims, k = 20_000, 64, 10
+    _case("ivf_device", rows=rows, dims=dims, k=k)
+    m, rng = _storage(orc, rows, dims, 99, mixture=True)
+    nq = 40
+    qs = orc.normalize_rows(rng.normal(size=(nq, dims)).astype(np.float32), nq, dims)
+    b = BackendIndex(m, approx=True)
+    ref = orc.Ivf(m)
+    ds = DeviceSearch(b.h, 64, 64)
+    dq = DevBuf.from_array(qs)
+    for nprobe in (1, 8, 32, ref.nlist):
+        didx, dsim, dlen = DevBuf(nq * k * 8), DevBuf(nq * k * 4), DevBuf(nq * 8)
+        ds.run_ivf(dq, nq, k, nprobe, didx, dsim, dlen)
+        idx = didx.to_array((nq, k), np.uint64)
+        sim = dsim.to_array((nq, k), np.float32)
+        lens = dlen.to_array((nq,), np.uint64)
+        for i in range(nq):
+            ridx, rsim = ref.search_gpu(qs[i], k, nprobe)
+            assert int(lens[i]) == len(ridx)
+            np.testing.assert_array_equal(idx[i, : lens[i]], ridx)
+            assert np.array_equal(sim[i, : lens[i]].view(np.uint32), rsim.view(np.uint32))
+        if nprobe == ref.nlist:
+            eidx, esim = b.search_batch(qs, k)[:2]
+            np.testing.assert_array_equal(idx, eidx)
+            assert np.array_equal(sim.view(np.uint32), esim.view(np.uint32))
+
+
+def test_ivf_fallbacks_and_frontend(orc):
+    """rows < 256 gives no IVF speedup: approx searches are exact
+    (src/gpu.rs:435-441).  An index created without approx ignores approx.
+    The front-end NNdex(approx=True) routes through the IVF path."""
+    _case("ivf_fallbacks")
+    m, rng = _storage(orc, 200, 16, 1)
+    q = orc.normalize_vector(rng.normal(size=16).astype(np.float32))
+    b = BackendIndex(m, approx=True)
+    idx, sim = b.search(q, 5, approx=True)
+    eidx, esim = b.search(q, 5)
+    np.testing.assert_array_equal(idx, eidx)
+    m2, rng = _storage(orc, 3000, 40, 2, mixture=True)
+    q2 = orc.normalize_vector(rng.normal(size=40).astype(np.float32))
+    plain = BackendIndex(m2, approx=False)
+    np.testing.assert_array_equal(plain.search(q2, 7, approx=True)[0], plain.search(q2, 7)[0])
+    raw = rng.normal(size=(3000, 40)).astype(np.float32)
+    ix = nndex_amd.NNdex(raw, approx=True, backend="gpu", enable_cache=False)
+    ref = orc.Ivf(orc.normalize_rows(raw, 3000, 40))
+    got_idx, got_sim = ix.search(q2[:40], k=9)
+    ridx, rsim = ref.search_gpu(orc.normalize_vector(q2[:40]), 9)
+    np.testing.assert_array_equal(got_idx, ridx)
+    assert np.array_equal(got_sim.view(np.uint32), rsim.view(np.uint32))
+
+
+def test_ivf_mixture_recall_and_structure(orc):
+    """A 200,000 x 128 Gaussian mixture (spread 0.3): size-independent checks
+    of the build (permutation, offsets, centroids = normalised means of their
+    rows) and recall@10 against the exact search at nprobe 8 / 32."""
+    rows, dims, k = 200_000, 128, 10
+    _case("ivf_mixture", rows=rows, dims=dims)
+    c = synth.mixture_centres(dims, 1024, seed=17)
+    m = synth.gaussian_mixture(rows, dims, c, 0.3, 17)
+    qs = synth.gaussian_mixture(256, dims, c, 0.3, 18)
+    b = BackendIndex(m, approx=True)
+    b.set_ivf(nlist=1024, iters=10)
+    info, cent, off, orig, assign = b.ivf_export()
+    print(f"build {info.build_ms:.1f} ms, rechecked {info.rechecked}", flush=True)
+    assert np.array_equal(np.sort(orig), np.arange(rows, dtype=np.uint64))
+    off64 = off.astype(np.int64)
+    assert off64[0] == 0 and off64[-1] == rows and np.all(np.diff(off64) >= 0)
+    assert np.array_equal(np.repeat(np.arange(1024), np.diff(off64)), assign[orig.astype(np.int64)])
+    # the last update: each centroid is the normalised f64 mean of its rows
+    for cl in range(0, 1024, 37):
+        members = orig[off64[cl]:off64[cl + 1]].astype(np.int64)
+        if len(members) == 0:
+            assert np.all(cent[cl] == 0)
+            continue
+        mean = m[members].astype(np.float64).mean(axis=0)
+        np.testing.assert_allclose(cent[cl], mean / np.linalg.norm(mean), atol=2e-7)
+    eidx = b.search_batch(qs, k)[0]
+    for nprobe, floor in ((8, 0.9), (32, 0.97)):
+        b.set_ivf(nprobe=nprobe)
+        aidx = b.search_batch(qs, k, approx=True)[0]
+        rec = np.mean([len(set(aidx[i]) & set(eidx[i])) / k for i in range(len(qs))])
+        print(f"nprobe {nprobe}: recall@10 {rec:.4f}", flush=True)
+        assert rec >= floor

@@ -371,6 +371,62 @@ def test_ivf_recall_floor(orc):
     assert int(approx[0]) in set(exact.tolist())
 
 
+def test_ivf_search_gpu_path(orc):
+    """The GPU backend's approximate search (src/gpu.rs:448-462).  Probing every
+    list scans every row once, so the result is the brute-force top-k of the
+    simsimd dot over all rows (ordered by score desc, then row asc); with one
+    probe every result comes from the cluster whose centroid scores highest."""
+    rng = np.random.default_rng(11)
+    rows, dims, k = 3000, 40, 10
+    m = orc.normalize_rows(rng.normal(size=(rows, dims)).astype(np.float32), rows, dims)
+    q = orc.normalize_vector(rng.normal(size=dims).astype(np.float32))
+    ivf = orc.Ivf(m)
+    assert ivf.nlist == orc.choose_nlist(rows) == 54
+    scores = np.array([orc.dot_product(m[r], q) for r in range(rows)], np.float32)
+    order = np.lexsort((np.arange(rows), -scores.astype(np.float64)))[:k]
+    idx, sim = ivf.search_gpu(q, k, nprobe=ivf.nlist)
+    assert idx.astype(np.int64).tolist() == order.tolist()
+    assert np.array_equal(sim, scores[order])
+    cscore = ivf.centroids().astype(np.float64) @ q.astype(np.float64)
+    best = int(np.argmax(cscore))
+    assert np.sort(cscore)[-1] - np.sort(cscore)[-2] > 1e-4  # no near-tie on this seed
+    off, perm = ivf.offsets(), ivf.original_indices()
+    members = set(perm[off[best]:off[best + 1]].tolist())
+    idx1, sim1 = ivf.search_gpu(q, k, nprobe=1)
+    assert len(idx1) == min(k, len(members)) and set(idx1.tolist()) <= members
+    assert np.all(np.diff(sim1) <= 0)
+    # nprobe = 0 is the reference heuristic
+    a, _ = ivf.search_gpu(q, k)
+    b, _ = ivf.search_gpu(q, k, nprobe=orc.choose_nprobe(ivf.nlist, k, rows))
+    assert np.array_equal(a, b)
+    # fewer candidates than k: every member returned
+    small = orc.Ivf(m, nlist=400)
+    off = small.offsets()
+    sizes = np.diff(off.astype(np.int64))
+    idx2, _ = small.search_gpu(q, 50, nprobe=1)
+    assert len(idx2) <= 50 and len(idx2) in set(sizes.tolist()) | {50}
+
+
+def test_ivf_kmeans_invariants(orc):
+    """spherical_kmeans (approx.rs:434-535) fixed points: each centroid is the
+    normalised f64 mean of its rows (zero when empty), and the assignment is
+    consistent with offsets and the permutation."""
+    rng = np.random.default_rng(5)
+    rows, dims = 2000, 24
+    m = orc.normalize_rows(rng.normal(size=(rows, dims)).astype(np.float32), rows, dims)
+    ivf = orc.Ivf(m, nlist=30, iters=3)
+    cent, assign = ivf.centroids(), ivf.assignments().astype(np.int64)
+    off, perm = ivf.offsets().astype(np.int64), ivf.original_indices().astype(np.int64)
+    for c in range(ivf.nlist):
+        rows_c = perm[off[c]:off[c + 1]]
+        assert np.all(assign[rows_c] == c) and np.all(np.diff(rows_c) > 0)
+        if len(rows_c) == 0:
+            assert np.all(cent[c] == 0)
+        else:
+            mean = m[rows_c].astype(np.float64).mean(axis=0)
+            np.testing.assert_allclose(cent[c], mean / np.linalg.norm(mean), atol=2e-7)
+
+
 def test_accuracy_vs_numpy_exact(orc):
     """tests_python/test_accuracy_vs_numpy.py:15-26: exact top-25 equals the
     numpy float32 reference; scores within 1e-5."""
