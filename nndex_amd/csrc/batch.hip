@@ -360,6 +360,7 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
         if constexpr (AM) {
             // top two over the 16 lanes that hold the same rows (one query
             // each); a pair closer than tol goes to the exact re-check
+            const float tol = *a.tol;
 #pragma unroll
             for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
                     const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
                     if (l16 == 0 && row < a.rows) {
                         a.assign[row] = key_row(b);
-                        if (c != 0 && !(key_sim(b) - key_sim(c) > a.tol))
+                        if (c != 0 && !(key_sim(b) - key_sim(c) > tol))
                             a.recheck[atomicAdd(a.recheck_count, 1u)] = row;
                     }
                 }
@@ -584,6 +585,34 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const float4 *__restri
     }
 }
 
+// Row-major rows -> BF16 units; one block per row.  err_sq: largest row norm
+// of the rounding error, f64 bits (non-negative doubles order as their bits).
+__global__ __launch_bounds__(256) void rows_to_bf16_kernel(const float *__restrict__ rows,
+                                                           uint32_t pdims,
+                                                           uint4 *__restrict__ out,
+                                                           unsigned long long *err_sq) {
+    __shared__ double red[256];
+    const uint32_t r = blockIdx.x, d8 = pdims / 8;
+    const float *src = rows + (size_t)r * pdims;
+    double ss = 0.0;
+    for (uint32_t u = threadIdx.x; u < d8; u += 256) {
+        const float *e = src + u * 8;
+        out[(size_t)r * d8 + u] = make_uint4(pack_bf16x2(e[0], e[1]), pack_bf16x2(e[2], e[3]),
+                                             pack_bf16x2(e[4], e[5]), pack_bf16x2(e[6], e[7]));
+        for (int j = 0; j < 8; ++j) {
+            const double d = (double)(float)(__bf16)e[j] - (double)e[j];
+            ss += d * d;
+        }
+    }
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(err_sq, __builtin_bit_cast(unsigned long long, red[0]));
+}
+
 // Graph-safe clear (a kernel, not a memset node).
 __global__ void zero_u32_kernel(uint32_t *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
@@ -609,6 +638,12 @@ void launch_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
     if (grid > 4096) grid = 4096;
     if (grid == 0) grid = 1;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(grid), dim3(256), 0, s, p, n);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_rows_to_bf16(const float *rows, uint32_t n, uint32_t pdims, uint4 *out,
+                         unsigned long long *err_sq, hipStream_t s) {
+    hipLaunchKernelGGL(rows_to_bf16_kernel, dim3(n), dim3(256), 0, s, rows, pdims, out, err_sq);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
@@ -673,7 +708,8 @@ void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream
     case STEPS:                                                                        \
         if (a.mode == 2 && !bf16)                                                      \
             return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT, true>(a, grid, s);        \
-        if (a.mode == 2) break;                                                        \
+        if (a.mode == 2)                                                               \
+            return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT, true>(a, grid, s);     \
         if (bf16)                                                                      \
             return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT>(a, grid, s);           \
         return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT>(a, grid, s);

@@ -23,10 +23,10 @@ struct MfmaScanArgs {
     float *out;                // [nq][ldo] scores (mode 1)
     uint32_t ldo;
     // mode 2: per row, the best query (ties -> lowest id); a row whose best
-    // and second-best approximate scores are within `tol` is appended to
+    // and second-best approximate scores are within *tol is appended to
     // `recheck` for an exact re-evaluation
     uint32_t *assign;          // [rows]
-    float tol;
+    const float *tol;          // device scalar
     uint32_t *recheck;         // [rows]
     uint32_t *recheck_count;
 };
@@ -65,5 +65,9 @@ void launch_rerank_full(const RerankArgs &a, uint32_t nq, hipStream_t s);
 void launch_gather_rows(const float4 *x, uint32_t d4, uint32_t stride, uint32_t rows_s,
                         float4 *dst, hipStream_t s);
 void launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+// Row-major FP32 rows -> row-major BF16 (16-byte units of 8), and the largest
+// per-row norm of the rounding error (f64 bits, atomic max into *err_sq).
+void launch_rows_to_bf16(const float *rows, uint32_t n, uint32_t pdims, uint4 *out,
+                         unsigned long long *err_sq, hipStream_t s);
 
 }  // namespace nndex

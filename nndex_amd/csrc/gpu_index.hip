@@ -325,13 +325,16 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
 void GpuIndex::search_exact_any(Workspace &ws, const float *dq, size_t nq, size_t k,
                                 uint64_t *d_idx, float *d_sim, hipStream_t s) {
     const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(k));
-    if (nq <= 8 || !mfma_supported(Precision::F32) || k > 400 || qbmax == 0) {
+    const bool f32 = mfma_supported(Precision::F32), b16 = mfma_supported(Precision::BF16);
+    if (nq <= 8 || !(f32 || b16) || k > 400 || qbmax == 0) {
         search_device(ws, dq, nq, k, d_idx, d_sim, s);
         return;
     }
+    // FP32 MFMA scoring up to 512 padded dims, BF16 above; the certificate
+    // and the scan fallback make either exact
     DeviceBuffer flags;
     flags.reserve(nq);
-    search_batch_mfma(ws, dq, nq, k, Precision::F32, 0, d_idx, d_sim,
+    search_batch_mfma(ws, dq, nq, k, f32 ? Precision::F32 : Precision::BF16, 0, d_idx, d_sim,
                       static_cast<uint8_t *>(flags.ptr), s);
     std::vector<uint8_t> h(nq);
     NNDEX_HIP_CHECK(hipMemcpyAsync(h.data(), flags.ptr, nq, hipMemcpyDeviceToHost, s));
@@ -704,11 +707,19 @@ const IvfIndex &GpuIndex::build_ivf() {
         ivf_ = std::move(ivf);
         return *ivf_;
     }
-    unsigned long long bits = 0;
+    unsigned long long bits = 0, ebits = 0;
     NNDEX_HIP_CHECK(hipMemcpy(&bits, xmax_sq_, sizeof(bits), hipMemcpyDeviceToHost));
     const double x_max = std::sqrt(__builtin_bit_cast(double, bits));
-    ivf_build(*ivf, x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(pdims_),
-              static_cast<uint32_t>(nlist), static_cast<uint32_t>(iters), x_max, ncu_, stream_);
+    double x_err = 0.0;
+    if (pdims_ > 512) {  // BF16 assignment
+        ensure_bf16(stream_);
+        NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
+        NNDEX_HIP_CHECK(hipMemcpy(&ebits, xerr_sq_ + 1, sizeof(ebits), hipMemcpyDeviceToHost));
+        x_err = std::sqrt(__builtin_bit_cast(double, ebits));
+    }
+    ivf_build(*ivf, x_, xbf16_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(pdims_),
+              static_cast<uint32_t>(nlist), static_cast<uint32_t>(iters), x_max, x_err, ncu_,
+              stream_);
     ivf->cindex = from_device_rows(ivf->centroids, nlist, pdims_, device_);
     ivf_ = std::move(ivf);
     return *ivf_;

@@ -15,7 +15,8 @@
 //     totals, LCG state = state * 6364136223846793005 + 1 seeded with
 //     rows * 2654435761 + dims, first row whose cumulative distance reaches
 //     state / u64::MAX * total, duplicate of the last centroid when total <= 0;
-//   * Lloyd iterations: assignment = FP32 MFMA GEMM of rows x centroids with
+//   * Lloyd iterations: assignment = MFMA GEMM of rows x centroids (FP32 up
+//     to 512 padded dims, BF16 copies above) with
 //     each row's two best centroids; a row whose two best approximate scores
 //     lie within the rounding bound is re-evaluated in the reference dot
 //     order, so the argmax equals the reference's (lowest id on ties);
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(256) void centroid_update_kernel(const float4 *__re
                                                               const uint32_t *__restrict__ sorted_rows,
                                                               const uint32_t *__restrict__ offsets,
                                                               float4 *__restrict__ centroids) {
-    __shared__ double scaled[512];
+    __shared__ double scaled[1024];
     __shared__ double nsq;
     const uint32_t cl = blockIdx.x, t = threadIdx.x;
     const uint32_t r0 = offsets[cl], r1 = offsets[cl + 1];
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float4 *__restrict__
                                                       const uint32_t *__restrict__ list,
                                                       const uint32_t *__restrict__ count,
                                                       uint32_t *__restrict__ assign) {
-    __shared__ float4 srow[128];
+    __shared__ float4 srow[256];
     __shared__ uint64_t red[4];
     const uint32_t n = *count;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
@@ -405,6 +406,30 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float4 *__restrict__
         best = block_max_u64(best, red);
         if (threadIdx.x == 0) assign[row] = key_row(best);
     }
+}
+
+// Re-check margin of the assignment.  The approximate (MFMA) score and the
+// reference dot lie within da and de of the exact dot:
+//   de = gamma_n |x| |c|                                  (four-lane FMA chain)
+//   da = gamma_n |x~| |c~| + |x~ - x| |c~| + |x| |c~ - c|  (x~, c~ = BF16 copies;
+//        for FP32 x~ = x, c~ = c and only the first term remains)
+// so a top-two gap above 2 (da + de) fixes the reference's argmax; the margin
+// used is twice that.  |x| <= x_max, |c| <= c_max, |x~ - x| <= x_err, and
+// |c~ - c| is the measured largest centroid rounding error.
+__global__ void assign_tol_kernel(const unsigned long long *__restrict__ cerr_sq, double x_err,
+                                  double x_max, double c_max, uint32_t pdims, int bf16,
+                                  float *__restrict__ tol) {
+    const double u = 0x1p-24, n = pdims;
+    const double gamma = n * u / (1.0 - n * u);
+    const double de = gamma * x_max * c_max;
+    double da = de;
+    if (bf16) {
+        const double ue = 0x1p-8;
+        const double xl = x_max * (1.0 + ue), cl = c_max * (1.0 + ue);
+        const double ec = sqrt(__builtin_bit_cast(double, *cerr_sq));
+        da = gamma * xl * cl + x_err * cl + x_max * ec;
+    }
+    *tol = (float)(4.0 * (da + de) * (1.0 + 1e-6));
 }
 
 // Reorder: reordered row j = row sorted[j] (both tiled, padding rows zero).
@@ -549,12 +574,16 @@ static uint32_t ivf_cw(uint32_t k) {
     return cw;
 }
 
-void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uint32_t nlist,
-               uint32_t iters, double x_max, uint32_t ncu, hipStream_t s) {
+void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows,
+               uint32_t pdims, uint32_t nlist, uint32_t iters, double x_max, double x_err,
+               uint32_t ncu, hipStream_t s) {
     const auto t0 = std::chrono::steady_clock::now();
-    const uint32_t d4 = pdims / 4;
-    if (pdims > 512)
-        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF build supports padded dims <= 512");
+    const uint32_t d4 = pdims / 4, d8 = pdims / 8;
+    const bool bf16 = pdims > 512;
+    if (pdims > 1024)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF build supports padded dims <= 1024");
+    if (bf16 && !xbf16)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the BF16 copy is required above 512 dims");
     if (nlist < 2 || nlist > rows || nlist > 16384)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "nlist must be in [2, min(rows, 16384)]");
     ivf.rows = rows;
@@ -575,12 +604,16 @@ void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, ui
     unsigned long long *rng = nullptr, *medoid = nullptr;
     uint32_t *sorted_keys = nullptr, *rows_iota = nullptr, *counts = nullptr, *recheck = nullptr,
              *recheck_count = nullptr;
+    uint4 *cbf16 = nullptr;
+    unsigned long long *cerr = nullptr;
+    float *tol = nullptr;
     void *sort_tmp = nullptr;
     size_t sort_bytes = 0;
     auto free_all = [&] {
         for (void *p : {(void *)partial, (void *)colpart, (void *)min_dist, (void *)mean,
                         (void *)rng, (void *)medoid, (void *)sorted_keys, (void *)rows_iota,
-                        (void *)counts, (void *)recheck, (void *)recheck_count, sort_tmp})
+                        (void *)counts, (void *)recheck, (void *)recheck_count, (void *)cbf16,
+                        (void *)cerr, (void *)tol, sort_tmp})
             if (p) (void)hipFree(p);
     };
     try {
@@ -595,6 +628,9 @@ void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, ui
         NNDEX_HIP_CHECK(hipMalloc(&counts, nlist * sizeof(uint32_t)));
         NNDEX_HIP_CHECK(hipMalloc(&recheck, (size_t)rows * sizeof(uint32_t)));
         NNDEX_HIP_CHECK(hipMalloc(&recheck_count, sizeof(uint32_t)));
+        NNDEX_HIP_CHECK(hipMalloc(&cerr, 2 * sizeof(unsigned long long)));
+        NNDEX_HIP_CHECK(hipMalloc(&tol, sizeof(float)));
+        if (bf16) NNDEX_HIP_CHECK(hipMalloc(&cbf16, (size_t)nlist * d8 * sizeof(uint4)));
         uint32_t bits = 1;
         while ((1u << bits) < nlist) ++bits;
         NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ivf.assign, sorted_keys,
@@ -624,36 +660,43 @@ void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, ui
         }
         NNDEX_HIP_CHECK(hipGetLastError());
 
-        // Lloyd iterations.  The MFMA pass and the reference dot each lie
-        // within gamma_n |x| |c| of the exact dot (|c| <= max(1, |x|max)), so
-        // a top-two gap above four such bounds fixes the reference's argmax;
-        // the margin used is twice that.
-        const double u = 0x1p-24, n = pdims;
-        const double gamma = n * u / (1.0 - n * u);
-        const double cmax = x_max > 1.0 ? x_max : 1.0;
-        const float tol = (float)(8.0 * gamma * x_max * cmax * (1.0 + 1e-6));
+        // Lloyd iterations: FP32 MFMA assignment up to 512 padded dims, BF16
+        // MFMA above (centroids rounded to BF16 each iteration); either way
+        // rows closer than the margin to a tie are re-checked exactly.
+        // Centroids are normalised means or rows: |c| <= max(1, x_max).
+        const double cmax = (x_max > 1.0 ? x_max : 1.0) * (1.0 + 1e-6);
+        launch_zero_u32(reinterpret_cast<uint32_t *>(cerr), 4, s);
+        if (!bf16)
+            hipLaunchKernelGGL(assign_tol_kernel, dim3(1), dim3(1), 0, s, cerr, 0.0, x_max, cmax,
+                               pdims, 0, tol);
         hipLaunchKernelGGL(iota_kernel, dim3(1024), dim3(256), 0, s, rows_iota, rows);
         const size_t hist_lds = (size_t)nlist * sizeof(uint32_t);
         std::vector<uint32_t> rechecked(iters, 0);
         for (uint32_t it = 0; it < iters; ++it) {
             launch_zero_u32(recheck_count, 1, s);
+            if (bf16) {
+                launch_zero_u32(reinterpret_cast<uint32_t *>(cerr), 2, s);
+                launch_rows_to_bf16(ivf.centroids, nlist, pdims, cbf16, cerr, s);
+                hipLaunchKernelGGL(assign_tol_kernel, dim3(1), dim3(1), 0, s, cerr, x_err, x_max,
+                                   cmax, pdims, 1, tol);
+            }
             MfmaScanArgs m{};
-            m.x = reinterpret_cast<const uint4 *>(x);
+            m.x = bf16 ? xbf16 : reinterpret_cast<const uint4 *>(x);
             m.rows = rows;
             m.rows_padded = static_cast<uint32_t>(prow);
-            m.dch = d4;
-            m.q = reinterpret_cast<const uint4 *>(ivf.centroids);
-            m.q_stride = d4;
+            m.dch = bf16 ? d8 : d4;
+            m.q = bf16 ? cbf16 : reinterpret_cast<const uint4 *>(ivf.centroids);
+            m.q_stride = m.dch;
             m.nq = nlist;
             m.mode = 2;
             m.assign = ivf.assign;
             m.tol = tol;
             m.recheck = recheck;
             m.recheck_count = recheck_count;
-            const uint32_t br = mfma_block_rows(false, m.dch);
+            const uint32_t br = mfma_block_rows(bf16, m.dch);
             uint32_t grid = (m.rows_padded + br - 1) / br;
             if (grid > ncu) grid = ncu;
-            launch_mfma_scan(m, false, grid, s);
+            launch_mfma_scan(m, bf16, grid, s);
             hipLaunchKernelGGL(recheck_kernel, dim3(ncu * 4), dim3(256), 0, s, x, d4,
                                reinterpret_cast<const float4 *>(ivf.centroids), nlist, recheck,
                                recheck_count, ivf.assign);

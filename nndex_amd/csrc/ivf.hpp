@@ -52,9 +52,11 @@ struct IvfScanArgs {
 };
 
 // Build the index of `rows` tiled rows (pdims padded columns) on stream s.
-// x_max is the largest row norm (bounds the assignment's rounding error).
-void ivf_build(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uint32_t nlist,
-               uint32_t iters, double x_max, uint32_t ncu, hipStream_t s);
+// x_max is the largest row norm; above 512 padded dims the assignment runs on
+// the BF16 copy xbf16, whose largest row rounding-error norm is x_err.
+void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows,
+               uint32_t pdims, uint32_t nlist, uint32_t iters, double x_max, double x_err,
+               uint32_t ncu, hipStream_t s);
 // Scan the probed lists and merge per query (probe ids already on device).
 void ivf_scan(const IvfScanArgs &a, uint32_t nq, hipStream_t s);
 

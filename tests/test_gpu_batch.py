@@ -50,6 +50,7 @@ def exact_device(b: BackendIndex, ds: DeviceSearch, qs: np.ndarray, k: int):
     (9_000, 64, 17, 1),
     (30_000, 96, 64, 100),
     (3_000, 8, 33, 20),
+    (12_000, 768, 24, 10),   # above 512 padded dims: BF16 scoring, certified or rescanned
 ])
 def test_exact_batch_through_frontend(orc, rows, dims, nq, k):
     """Batches above 8 queries take the certified MFMA path; every result must

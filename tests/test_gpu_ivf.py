@@ -44,6 +44,8 @@ def _storage(orc, rows, dims, seed, mixture=False):
     (4096, 128, 64, 6),
     (2500, 8, 16, 3),
     (6000, 32, 300, 2),    # many small clusters, some may empty
+    (3000, 600, 0, 3),     # above 512 padded dims: BF16 assignment + exact re-check
+    (2000, 1000, 40, 2),
 ])
 def test_ivf_build_equals_oracle(orc, rows, dims, nlist, iters):
     _case("ivf_build", rows=rows, dims=dims, nlist=nlist, iters=iters)
