@@ -15,6 +15,11 @@ Also reported:
                 (ParallelFold strategy, all threads) on a bounded sample;
   batch         QPS at batch sizes 16 / 256 / 4096 (exact FP32) and the BF16
                 scoring + exact FP32 rerank form at 4096;
+  serve         the serving form: a resident 2,000,000 x 1,024 index (FP32 +
+                BF16 copies, IVF built), Poisson arrivals at a fixed rate, per
+                request B log-uniform in 1..2048 rows and k in {1, 10, 100},
+                half exact and half IVF (nprobe 32), served by the continuous-
+                batching loop: p50/p99 latency and throughput;
   ivf           the IVF form: 10,000,000 x 128 Gaussian mixture (4,096
                 centres, spread 0.3), k-means with nlist 4,096 and 20
                 iterations built on the GPU, 1,024 queries at nprobe 8 / 32 /
@@ -155,6 +160,90 @@ def ivf_bench(args, k: int = 10) -> dict:
     return res
 
 
+def serve_bench(args) -> dict:
+    """The serving form (see the module docstring): open-loop Poisson arrivals
+    into the continuous-batching loop over a resident index."""
+    import threading
+
+    from nndex_amd.serve import EXACT, IVF, Server
+
+    L = _lib.lib()
+    rows, dims = args.serve_rows, args.serve_dims
+    t0 = time.time()
+    base, bank = synth.mixture_workload(rows=rows, dims=dims, n_queries=8192)
+    log(f"serve workload {rows}x{dims} generated in {time.time() - t0:.1f}s")
+    h = ctypes.c_void_p()
+    check(L.nndex_gpu_index_new(base.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rows, dims,
+                                1, 0, -1, ctypes.byref(h)))
+    del base
+    info = _lib.IvfInfo()
+    t0 = time.perf_counter()
+    check(L.nndex_gpu_index_build_ivf(h, ctypes.byref(info)))
+    log(f"serve: IVF nlist {info.nlist} built in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    srv = Server(h, dims, rows, slots=args.serve_slots, max_batch_queries=2048, max_k=100,
+                 ivf_nprobe_max=32)
+    st0 = srv.stats()
+    log(f"serve: {st0['slots']} executors of {st0['slot_bytes'] / 2**30:.2f} GiB prepared in "
+        f"{time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(2024)
+    n_req = max(1, int(args.serve_rate * args.serve_seconds))
+    gaps = rng.exponential(1.0 / args.serve_rate, size=n_req)
+    sizes = np.clip(np.round(np.exp(rng.uniform(0.0, np.log(2048), size=n_req))), 1,
+                    2048).astype(int)
+    ks = rng.choice([1, 10, 100], size=n_req)
+    modes = rng.integers(0, 2, size=n_req)
+    starts = rng.integers(0, bank.shape[0] - 2048, size=n_req)
+    # warm-up: one request of each kind
+    for m in (EXACT, IVF):
+        srv.wait(srv.submit(bank[:64], 10, m, 32 if m == IVF else 0))
+    srv.reset_stats()
+    tickets = []
+    lock = threading.Lock()
+    t_start = time.perf_counter()
+    due = t_start
+    for i in range(n_req):
+        due += gaps[i]
+        now = time.perf_counter()
+        if due > now:
+            time.sleep(due - now)
+        q = bank[starts[i]:starts[i] + sizes[i]]
+        m = int(modes[i])
+        t = srv.submit(q, int(ks[i]), m, 32 if m == IVF else 0)
+        with lock:
+            tickets.append((t, m, int(sizes[i])))
+        if i % max(1, n_req // 10) == 0:
+            log(f"serve: submitted {i + 1}/{n_req}")
+    offered_s = time.perf_counter() - t_start
+    lat = {EXACT: [], IVF: []}
+    for t, m, _ in tickets:
+        lat[m].append(srv.wait(t)[3])
+    st = srv.stats()
+    srv.close()
+    L.nndex_gpu_index_free(h)
+    res = {"workload": f"{rows}x{dims} Gaussian mixture (1000 centres, spread 0.5, "
+                       "unit-normalised), FP32 + BF16 copies, IVF nlist "
+                       f"{int(info.nlist)}; Poisson arrivals at {args.serve_rate:g}/s for "
+                       f"{args.serve_seconds:g} s; rows per request log-uniform 1..2048, "
+                       "k in {1,10,100}, half exact / half IVF nprobe 32",
+           "requests": int(st["requests"]), "queries": int(st["queries"]),
+           "offered_rows_per_s": round(float(sizes.sum()) / offered_s, 1),
+           "queries_per_s": round(st["queries_per_s"], 1),
+           "requests_per_s": round(st["requests_per_s"], 2),
+           "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+           "max_ms": round(st["max_ms"], 3),
+           "exact_p50_ms": round(float(np.percentile(lat[EXACT], 50)), 3) if lat[EXACT] else None,
+           "exact_p99_ms": round(float(np.percentile(lat[EXACT], 99)), 3) if lat[EXACT] else None,
+           "ivf_p50_ms": round(float(np.percentile(lat[IVF], 50)), 3) if lat[IVF] else None,
+           "ivf_p99_ms": round(float(np.percentile(lat[IVF], 99)), 3) if lat[IVF] else None,
+           "batches": int(st["batches"]),
+           "mean_batch_queries": round(st["mean_batch_queries"], 1),
+           "executors": int(st["slots"]),
+           "executor_bytes": int(st["slot_bytes"])}
+    log(f"serve: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +268,15 @@ def main():
     ap.add_argument("--ivf-nlist", type=int, default=4096)
     ap.add_argument("--ivf-iters", type=int, default=20)
     ap.add_argument("--ivf-queries", type=int, default=1024)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving form and print its record")
+    ap.add_argument("--serve-rows", type=int, default=2_000_000)
+    ap.add_argument("--serve-dims", type=int, default=1024)
+    ap.add_argument("--serve-rate", type=float, default=50.0,
+                    help="request arrivals per second (Poisson)")
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: copy calibration + a few searches only")
     args = ap.parse_args()
@@ -190,6 +288,9 @@ def main():
     L = _lib.lib()
     if args.ivf_only:
         print(json.dumps({"ivf": ivf_bench(args)}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serve": serve_bench(args)}), flush=True)
         return
     rows, dims, k = args.rows, args.dims, args.k
     t0 = time.time()
@@ -380,6 +481,7 @@ def main():
     L.nndex_gpu_index_free(h)
     del base
     ivf = None if args.no_ivf else ivf_bench(args)
+    serve = None if args.no_serve else serve_bench(args)
 
     out = {
         "metric": METRIC,
@@ -409,6 +511,7 @@ def main():
         "autotune": tune.strip().splitlines()[-1] if tune else None,
         "batch": batch,
         "ivf": ivf,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

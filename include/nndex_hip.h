@@ -224,6 +224,57 @@ nndex_status nndex_gpu_index_search_ivf_device(nndex_gpu_index *index, nndex_wor
                                                float *device_similarities,
                                                uint64_t *device_lens, void *stream);
 
+/* ---- Serving loop (no reference counterpart: the reference serves one  */
+/* call at a time behind a Mutex, src/gpu.rs:511-592) ---------------------- */
+
+/* Continuous batching over one resident index.  Requests (normalised padded
+ * query rows, k, mode 0 = exact / 1 = IVF with nprobe) are queued; each of
+ * `slots` executors (own stream, workspace and staging, sized once for
+ * max_batch_queries x max_k) takes the oldest pending request plus every
+ * later one of the same kind that fits and runs them as one batch at their
+ * largest k, so work arriving while batches run is admitted as executors free
+ * and overlaps on the device.  Admission by HBM footprint: the number of
+ * executors is the largest whose fixed footprint fits hbm_budget_bytes;
+ * nothing is allocated while serving.  A request's results equal those of the
+ * same request served alone. */
+typedef struct nndex_server_options {
+    uint32_t slots;             /* executors in flight; 0 = 4 */
+    uint32_t max_batch_queries; /* rows per batch; 0 = 2048 (larger requests are split) */
+    uint32_t max_k;             /* largest k served; 0 = 100 */
+    uint32_t ivf_nprobe_max;    /* 0 = exact requests only; else IVF up to this nprobe */
+    uint64_t hbm_budget_bytes;  /* device bytes for all executors; 0 = 16 GiB */
+} nndex_server_options;
+
+typedef struct nndex_server_stats {
+    uint64_t slots;             /* executors admitted by the budget */
+    uint64_t slot_bytes;        /* device footprint of one executor */
+    uint64_t requests;          /* completed since the last reset */
+    uint64_t queries;           /* rows of those requests */
+    uint64_t batches;
+    double mean_batch_queries;
+    double p50_ms, p99_ms, max_ms;  /* submit -> results ready, per request */
+    double span_s;              /* first submit -> last completion */
+    double queries_per_s, requests_per_s;
+} nndex_server_stats;
+
+typedef struct nndex_server nndex_server;
+
+nndex_status nndex_server_new(nndex_gpu_index *index, const nndex_server_options *options,
+                              nndex_server **out);
+/* Stops the executors after their current batch; pending requests are dropped. */
+void nndex_server_free(nndex_server *server);
+/* Queue `query_rows` normalised padded rows; returns a ticket at once.
+ * nprobe = 0 picks the reference's choose_nprobe for k (IVF only). */
+nndex_status nndex_server_submit(nndex_server *server, const float *queries, size_t query_rows,
+                                 size_t k, int32_t mode, size_t nprobe, uint64_t *ticket);
+/* Block until the ticket's request is done; copy query_rows x min(k, rows)
+ * indices and similarities and query_rows result counts (any may be NULL). */
+nndex_status nndex_server_wait(nndex_server *server, uint64_t ticket, uint64_t *out_indices,
+                               float *out_similarities, uint64_t *out_lens,
+                               double *latency_ms);
+nndex_status nndex_server_get_stats(nndex_server *server, nndex_server_stats *out);
+nndex_status nndex_server_reset_stats(nndex_server *server);
+
 /* ======================================================================== */
 /* 3. Front-end NNdex (src/lib.rs) in C++                                   */
 /* ======================================================================== */

@@ -106,6 +106,40 @@ class IvfInfo(ctypes.Structure):
     ]
 
 
+class ServerOptions(ctypes.Structure):
+    """nndex_server_options."""
+
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("max_batch_queries", ctypes.c_uint32),
+        ("max_k", ctypes.c_uint32),
+        ("ivf_nprobe_max", ctypes.c_uint32),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    """nndex_server_stats."""
+
+    _fields_ = [
+        ("slots", ctypes.c_uint64),
+        ("slot_bytes", ctypes.c_uint64),
+        ("requests", ctypes.c_uint64),
+        ("queries", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("mean_batch_queries", ctypes.c_double),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("span_s", ctypes.c_double),
+        ("queries_per_s", ctypes.c_double),
+        ("requests_per_s", ctypes.c_double),
+    ]
+
+    def to_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
 _I32 = ctypes.c_int32
@@ -138,6 +172,13 @@ SIGNATURES = {
     "nndex_gpu_index_build_ivf": (_ST, [_P, ctypes.POINTER(IvfInfo)]),
     "nndex_gpu_index_ivf_export": (_ST, [_P, _FP, _U64P, _U64P, _U64P]),
     "nndex_gpu_index_search_ivf_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _P, _P]),
+    "nndex_server_new": (_ST, [_P, ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
+    "nndex_server_free": (None, [_P]),
+    "nndex_server_submit": (_ST, [_P, _FP, _SZ, _SZ, _I32, _SZ, _U64P]),
+    "nndex_server_wait": (_ST, [_P, ctypes.c_uint64, _U64P, _FP, _U64P,
+                                ctypes.POINTER(ctypes.c_double)]),
+    "nndex_server_get_stats": (_ST, [_P, ctypes.POINTER(ServerStats)]),
+    "nndex_server_reset_stats": (_ST, [_P]),
     "nndex_device_copy": (_ST, [_P, _P, _SZ, _P]),
     "nndex_new": (_ST, [_FP, _SZ, _SZ, _SZ, ctypes.POINTER(IndexOptions), ctypes.POINTER(_P)]),
     "nndex_free": (None, [_P]),

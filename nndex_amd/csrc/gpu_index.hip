@@ -47,7 +47,8 @@ PinnedBuffer::~PinnedBuffer() {
 Workspace::~Workspace() {
     for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
                     (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
-                    (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part})
+                    (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
+                    (void *)xflags})
         if (p) (void)hipFree(p);
 }
 
@@ -206,6 +207,8 @@ void GpuIndex::ensure_bf16(hipStream_t s) {
     NNDEX_HIP_CHECK(hipMemsetAsync(xerr_sq_, 0, 2 * sizeof(unsigned long long), s));
     launch_bf16_error(x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(d4_), xerr_sq_ + 1,
                       s);
+    // complete before the lock is released: another stream may use it next
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s) {
@@ -221,11 +224,13 @@ const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s)
         const size_t prows = padded_rows(hit->rows);
         NNDEX_HIP_CHECK(hipMalloc(&hit->f32, prows * d4_ * sizeof(float4)));
         launch_gather_rows(x_, static_cast<uint32_t>(d4_), stride, hit->rows, hit->f32, s);
+        NNDEX_HIP_CHECK(hipStreamSynchronize(s));
     }
     if (bf16 && !hit->bf16) {
         const size_t prows = padded_rows(hit->rows);
         NNDEX_HIP_CHECK(hipMalloc(&hit->bf16, prows * (pdims_ / 8) * sizeof(uint4)));
         launch_to_bf16(hit->f32, prows, static_cast<uint32_t>(pdims_ / 8), hit->bf16, s);
+        NNDEX_HIP_CHECK(hipStreamSynchronize(s));
     }
     return *hit;
 }
@@ -248,20 +253,12 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
     if (!mfma_supported(p))
         throw Error(NNDEX_ERR_UNSUPPORTED, "dims too large for the MFMA batch path");
     if (k == 0 || k > rows_) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
-    if (kp == 0) kp = default_overfetch(k, p);
-    if (kp < k) kp = k;
-    if (kp > rows_) kp = rows_;
-    if (kp > 447) throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 447 is not supported");
+    const BatchPlan plan = plan_batch(k, p, kp);
+    kp = plan.kp;
     if (bf16) ensure_bf16(s);
-    // sample: about max(256 k', rows/16) rows -> about 16 k' candidates
-    size_t target = 256 * kp > rows_ / 16 ? 256 * kp : rows_ / 16;
-    uint32_t stride = static_cast<uint32_t>(rows_ / (target ? target : 1));
-    if (stride < 1) stride = 1;
-    if (stride > 64) stride = 64;
-    const RowSample &smp = sample_for(stride, bf16, s);
-    size_t capq = 8 * kp * stride;
-    if (capq < 8192) capq = 8192;
-    const size_t group = 2048;
+    const RowSample &smp = sample_for(plan.stride, bf16, s);
+    const size_t capq = plan.capq;
+    const size_t group = plan.group;
     const uint32_t dch = static_cast<uint32_t>(bf16 ? pdims_ / 8 : pdims_ / 4);
     for (size_t q0 = 0; q0 < nq; q0 += group) {
         const size_t n = nq - q0 < group ? nq - q0 : group;
@@ -322,6 +319,105 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
     }
 }
 
+BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
+    BatchPlan b;
+    if (kp == 0) kp = default_overfetch(k, p);
+    if (kp < k) kp = k;
+    if (kp > rows_) kp = rows_;
+    if (kp > 447) throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 447 is not supported");
+    b.kp = kp;
+    // sample: about max(256 k', rows/16) rows -> about 16 k' candidates
+    const size_t target = 256 * kp > rows_ / 16 ? 256 * kp : rows_ / 16;
+    size_t stride = rows_ / (target ? target : 1);
+    if (stride < 1) stride = 1;
+    if (stride > 64) stride = 64;
+    b.stride = static_cast<uint32_t>(stride);
+    b.rows_s = (rows_ + stride - 1) / stride;
+    b.capq = 8 * kp * stride;
+    if (b.capq < 8192) b.capq = 8192;
+    return b;
+}
+
+size_t BatchPlan::workspace_bytes(size_t nq, size_t pdims) const {
+    const size_t n = nq < group ? nq : group;
+    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 1) + n * rows_s * 4 + n * capq * 8;
+}
+
+bool GpuIndex::exact_batch_precision(Precision &p) const {
+    if (mfma_supported(Precision::F32)) {
+        p = Precision::F32;
+        return true;
+    }
+    if (mfma_supported(Precision::BF16)) {
+        p = Precision::BF16;
+        return true;
+    }
+    return false;
+}
+
+void Workspace::reserve_xflags(size_t nq) {
+    if (nq <= xflags_n) return;
+    if (xflags) NNDEX_HIP_CHECK(hipFree(xflags));
+    xflags = nullptr;
+    NNDEX_HIP_CHECK(hipMalloc(&xflags, nq + 256));
+    xflags_n = nq;
+}
+
+void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hipStream_t s) {
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    if (k > ws.max_k) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k exceeds the workspace's max_k");
+    Precision p;
+    if (nq > 8 && exact_batch_precision(p)) {
+        // every k up to the maximum: the sample stride and the candidate
+        // capacity are not monotone in k
+        const bool bf16 = p == Precision::BF16;
+        if (bf16) ensure_bf16(s);
+        size_t rows_s = 0, capq = 0, group = 0;
+        for (size_t kk = 1; kk <= k && kk <= 400 && kk <= rows_; ++kk) {
+            const BatchPlan plan = plan_batch(kk, p, 0);
+            sample_for(plan.stride, bf16, s);
+            rows_s = plan.rows_s > rows_s ? plan.rows_s : rows_s;
+            capq = plan.capq > capq ? plan.capq : capq;
+            group = plan.group;
+        }
+        if (group) ws.reserve_batch(nq < group ? nq : group, pdims_, rows_s, capq);
+        ws.reserve_xflags(nq);
+    }
+    if (ivf_nprobe) {
+        const IvfIndex &ivf = build_ivf();
+        if (ivf.nlist >= 2) {
+            const size_t np = ivf_nprobe < ivf.nlist ? ivf_nprobe : ivf.nlist;
+            ws.reserve_ivf(nq, np, k);
+            if (!ws.sub || ws.sub->max_k < np)
+                ws.sub = ivf.cindex->make_workspace(8, np < 10 ? 10 : np);
+            ivf.cindex->prepare(*ws.sub, nq, np, 0, s);
+        }
+    }
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+size_t GpuIndex::workspace_bytes(size_t nq, size_t k, size_t ivf_nprobe) {
+    size_t bytes = static_cast<size_t>(ncu_) * 8 * (2048 * 8 + 8 * 4) + 256;
+    Precision p;
+    if (nq > 8 && exact_batch_precision(p)) {
+        size_t most = 0;
+        for (size_t kk = 1; kk <= k && kk <= 400 && kk <= rows_; ++kk) {
+            const size_t b = plan_batch(kk, p, 0).workspace_bytes(nq, pdims_);
+            most = b > most ? b : most;
+        }
+        bytes += most + nq;
+    }
+    if (ivf_nprobe) {
+        const IvfIndex &ivf = build_ivf();
+        if (ivf.nlist >= 2) {
+            const size_t np = ivf_nprobe < ivf.nlist ? ivf_nprobe : ivf.nlist;
+            bytes += nq * np * (8 + 4 + 8 * k);
+            bytes += ivf.cindex->workspace_bytes(nq, np, 0);
+        }
+    }
+    return bytes;
+}
+
 void GpuIndex::search_exact_any(Workspace &ws, const float *dq, size_t nq, size_t k,
                                 uint64_t *d_idx, float *d_sim, hipStream_t s) {
     const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(k));
@@ -332,12 +428,11 @@ void GpuIndex::search_exact_any(Workspace &ws, const float *dq, size_t nq, size_
     }
     // FP32 MFMA scoring up to 512 padded dims, BF16 above; the certificate
     // and the scan fallback make either exact
-    DeviceBuffer flags;
-    flags.reserve(nq);
+    ws.reserve_xflags(nq);
     search_batch_mfma(ws, dq, nq, k, f32 ? Precision::F32 : Precision::BF16, 0, d_idx, d_sim,
-                      static_cast<uint8_t *>(flags.ptr), s);
+                      ws.xflags, s);
     std::vector<uint8_t> h(nq);
-    NNDEX_HIP_CHECK(hipMemcpyAsync(h.data(), flags.ptr, nq, hipMemcpyDeviceToHost, s));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(h.data(), ws.xflags, nq, hipMemcpyDeviceToHost, s));
     NNDEX_HIP_CHECK(hipStreamSynchronize(s));
     for (size_t q = 0; q < nq; ++q)
         if (h[q]) search_device(ws, dq + q * pdims_, 1, k, d_idx + q * k, d_sim + q * k, s);

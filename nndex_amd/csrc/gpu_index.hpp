@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -56,7 +57,23 @@ struct Workspace {
     uint64_t *ivf_part = nullptr;
     std::unique_ptr<Workspace> sub;
     void reserve_ivf(size_t nq, size_t nprobe, size_t k);
+    // certificate flags of the exact batch path
+    uint8_t *xflags = nullptr;
+    size_t xflags_n = 0;
+    void reserve_xflags(size_t nq);
     ~Workspace();
+};
+
+// Sizes of the batched MFMA path for one (k, precision): over-fetch k',
+// sample stride and rows, candidate capacity per query, queries per group.
+struct BatchPlan {
+    size_t kp = 0;
+    uint32_t stride = 1;
+    size_t rows_s = 0;
+    size_t capq = 0;
+    size_t group = 2048;
+    // device bytes of the workspace buffers for nq queries
+    size_t workspace_bytes(size_t nq, size_t pdims) const;
 };
 
 // Strided row sample used by the batched path's threshold stage.
@@ -130,6 +147,16 @@ class GpuIndex {
                           float *d_sim, hipStream_t s);
     bool mfma_supported(Precision p) const;
     static size_t default_overfetch(size_t k, Precision p);
+    BatchPlan plan_batch(size_t k, Precision p, size_t kp) const;
+    // Precision of the exact batch path (FP32 MFMA up to 512 padded dims,
+    // BF16 above); false when neither MFMA form applies (fused scan only).
+    bool exact_batch_precision(Precision &p) const;
+    // Allocate everything a later search of up to nq queries and k results
+    // needs (lazy copies, row sample, workspace buffers; IVF index and its
+    // centroid workspace when ivf_nprobe > 0), so it allocates nothing.
+    void prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hipStream_t s);
+    // Device bytes prepare() reserves in a workspace for nq queries, k results.
+    size_t workspace_bytes(size_t nq, size_t k, size_t ivf_nprobe);
 
     // ---- IVF prefilter (src/approx.rs through src/gpu.rs:435-462) ----
     // 0 keeps the reference's rule (choose_nlist, KMEANS_MAX_ITERS,
@@ -182,7 +209,7 @@ class GpuIndex {
     unsigned long long *xmax_sq_ = nullptr;
     unsigned long long *xerr_sq_ = nullptr;  // [0] unused, [1] BF16 rounding error
     std::mutex lazy_mu_;
-    std::vector<RowSample> samples_;
+    std::deque<RowSample> samples_;  // references stay valid as it grows
     const RowSample &sample_for(uint32_t stride, bool bf16, hipStream_t s);
     void ensure_bf16(hipStream_t s);
     hipStream_t stream_ = nullptr;

@@ -115,3 +115,19 @@ def test_io_loaders(tmp_path):
     assert np.array_equal(load_matrix_from_disk(str(tmp_path / "m.parquet"), "v"), m)
     with pytest.raises(DiskLoadError, match="was not found"):
         load_matrix_from_disk(str(tmp_path / "m.parquet"), "w")
+
+
+def test_server_boundary_checks_without_gpu():
+    """Argument checks of the serving ABI run before any device call."""
+    import ctypes
+
+    from nndex_amd import _lib
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.nndex_server_new(None, None, ctypes.byref(h)) == 20
+    assert b"null pointer" in L.nndex_last_error()
+    t = ctypes.c_uint64()
+    assert L.nndex_server_submit(None, None, 1, 1, 0, 0, ctypes.byref(t)) == 20
+    assert L.nndex_server_wait(None, 1, None, None, None, None) == 20
+    L.nndex_server_free(None)
