@@ -48,7 +48,7 @@ Workspace::~Workspace() {
     for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
                     (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
                     (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
-                    (void *)xflags})
+                    (void *)xflags, (void *)ivf_pair_u32, (void *)ivf_list_u32, ivf_sort_tmp})
         if (p) (void)hipFree(p);
 }
 
@@ -80,15 +80,26 @@ void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t cap
     }
 }
 
-void Workspace::reserve_ivf(size_t nq, size_t nprobe, size_t k) {
-    if (nq * nprobe > ivf_probe_elems) {
-        regrow(ivf_probe, nq * nprobe);
-        regrow(ivf_probe_sim, nq * nprobe);
-        ivf_probe_elems = nq * nprobe;
+void Workspace::reserve_ivf(size_t pairs, size_t nlist, size_t part_elems, size_t sort_bytes) {
+    if (pairs > ivf_pairs) {
+        regrow(ivf_probe, pairs);
+        regrow(ivf_probe_sim, pairs);
+        regrow(ivf_pair_u32, 4 * pairs);
+        ivf_pairs = pairs;
     }
-    if (nq * nprobe * k > ivf_part_elems) {
-        regrow(ivf_part, nq * nprobe * k);
-        ivf_part_elems = nq * nprobe * k;
+    if (nlist > ivf_lists) {
+        regrow(ivf_list_u32, 4 * (nlist + 1));
+        ivf_lists = nlist;
+    }
+    if (part_elems > ivf_part_elems) {
+        regrow(ivf_part, part_elems);
+        ivf_part_elems = part_elems;
+    }
+    if (sort_bytes > ivf_sort_bytes) {
+        if (ivf_sort_tmp) NNDEX_HIP_CHECK(hipFree(ivf_sort_tmp));
+        ivf_sort_tmp = nullptr;
+        NNDEX_HIP_CHECK(hipMalloc(&ivf_sort_tmp, sort_bytes));
+        ivf_sort_bytes = sort_bytes;
     }
 }
 
@@ -387,7 +398,18 @@ void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hi
         const IvfIndex &ivf = build_ivf();
         if (ivf.nlist >= 2) {
             const size_t np = ivf_nprobe < ivf.nlist ? ivf_nprobe : ivf.nlist;
-            ws.reserve_ivf(nq, np, k);
+            // every (batch size, nprobe, k) up to the maximum
+            size_t part = 0;
+            for (size_t n = 1; n <= nq; n = n < 16 ? n + 1 : (2 * n < nq || n == nq ? 2 * n : nq))
+                for (size_t p = 1; p <= np; ++p)
+                    for (size_t kk : {(size_t)1, (size_t)10, k}) {
+                        if (kk > k) continue;
+                        uint32_t qb, nchunk;
+                        ivf_scan_shape(n, p, ivf.nlist, rows_, kk, qb, nchunk);
+                        const size_t e = n * p * nchunk * kk;
+                        part = e > part ? e : part;
+                    }
+            ws.reserve_ivf(nq * np, ivf.nlist, part, ivf_sort_bytes(nq * np, ivf.nlist));
             if (!ws.sub || ws.sub->max_k < np)
                 ws.sub = ivf.cindex->make_workspace(8, np < 10 ? 10 : np);
             ivf.cindex->prepare(*ws.sub, nq, np, 0, s);
@@ -411,7 +433,7 @@ size_t GpuIndex::workspace_bytes(size_t nq, size_t k, size_t ivf_nprobe) {
         const IvfIndex &ivf = build_ivf();
         if (ivf.nlist >= 2) {
             const size_t np = ivf_nprobe < ivf.nlist ? ivf_nprobe : ivf.nlist;
-            bytes += nq * np * (8 + 4 + 8 * k);
+            bytes += nq * np * (8 + 4) + ivf_scan_scratch_bytes(nq, np, ivf.nlist, rows_, k);
             bytes += ivf.cindex->workspace_bytes(nq, np, 0);
         }
     }
@@ -837,7 +859,10 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     if (nprobe == 0) nprobe = ivf_nprobe_;
     if (nprobe == 0) nprobe = ivf_choose_nprobe(ivf.nlist, k, rows_);
     if (nprobe > ivf.nlist) nprobe = ivf.nlist;
-    ws.reserve_ivf(nq, nprobe, k);
+    uint32_t qb, nchunk;
+    ivf_scan_shape(nq, nprobe, ivf.nlist, rows_, k, qb, nchunk);
+    const size_t pairs = nq * nprobe;
+    ws.reserve_ivf(pairs, ivf.nlist, pairs * nchunk * k, ivf_sort_bytes(pairs, ivf.nlist));
     if (!ws.sub || ws.sub->max_k < nprobe)
         ws.sub = ivf.cindex->make_workspace(8, nprobe < 10 ? 10 : nprobe);
     // candidate_indices (approx.rs:110-138): top-nprobe centroids by score
@@ -847,16 +872,30 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     a.d4 = static_cast<uint32_t>(d4_);
     a.q = dq;
     a.probe = ws.ivf_probe;
-    a.probe_stride = static_cast<uint32_t>(nprobe);
+    a.nq = static_cast<uint32_t>(nq);
     a.nprobe = static_cast<uint32_t>(nprobe);
+    a.nlist = ivf.nlist;
+    a.k = static_cast<uint32_t>(k);
+    a.qb = qb;
+    a.nchunk = nchunk;
     a.offsets = ivf.offsets;
     a.orig = ivf.orig;
-    a.k = static_cast<uint32_t>(k);
+    a.pair_key = ws.ivf_pair_u32;
+    a.pair_val = ws.ivf_pair_u32 + pairs;
+    a.sorted_key = ws.ivf_pair_u32 + 2 * pairs;
+    a.sorted_pair = ws.ivf_pair_u32 + 3 * pairs;
+    const size_t L1 = ivf.nlist + 1;
+    a.pair_cnt = ws.ivf_list_u32;
+    a.pair_off = ws.ivf_list_u32 + L1;
+    a.grp_cnt = ws.ivf_list_u32 + 2 * L1;
+    a.grp_off = ws.ivf_list_u32 + 3 * L1;
     a.part = ws.ivf_part;
+    a.sort_tmp = ws.ivf_sort_tmp;
+    a.sort_bytes = ws.ivf_sort_bytes;
     a.out_idx = d_idx;
     a.out_sim = d_sim;
     a.out_len = d_len;
-    ivf_scan(a, static_cast<uint32_t>(nq), s);
+    ivf_scan(a, s);
 }
 
 }  // namespace nndex

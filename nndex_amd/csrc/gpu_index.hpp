@@ -49,14 +49,17 @@ struct Workspace {
     uint32_t *cand_count = nullptr;
     uint8_t *flags = nullptr;
     void reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq);
-    // IVF search: probed cluster ids and scores, per-list keys, and the
-    // centroid index's own workspace
-    size_t ivf_probe_elems = 0, ivf_part_elems = 0;
+    // IVF search: probed cluster ids and scores, the list scan's pair sort
+    // and partial keys, and the centroid index's own workspace
+    size_t ivf_pairs = 0, ivf_lists = 0, ivf_part_elems = 0, ivf_sort_bytes = 0;
     uint64_t *ivf_probe = nullptr;
     float *ivf_probe_sim = nullptr;
+    uint32_t *ivf_pair_u32 = nullptr;  // [4][pairs]: key, val, sorted key, sorted pair
+    uint32_t *ivf_list_u32 = nullptr;  // [4][lists + 1]: pair count/offset, group count/offset
     uint64_t *ivf_part = nullptr;
+    void *ivf_sort_tmp = nullptr;
     std::unique_ptr<Workspace> sub;
-    void reserve_ivf(size_t nq, size_t nprobe, size_t k);
+    void reserve_ivf(size_t pairs, size_t nlist, size_t part_elems, size_t sort_bytes);
     // certificate flags of the exact batch path
     uint8_t *xflags = nullptr;
     size_t xflags_n = 0;

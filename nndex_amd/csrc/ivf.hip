@@ -454,50 +454,102 @@ __global__ __launch_bounds__(256) void gather_sorted_kernel(const float4 *__rest
 
 // ---- search -----------------------------------------------------------------
 
-// Block (query q, probe p): the list's rows of the reordered copy scored in
-// the reference dot order; the best k keys (original row ids) per list.
-template <int CW>
-__global__ __launch_bounds__(256) void ivf_scan_kernel(IvfScanArgs a) {
-    __shared__ uint64_t slots[4][CW];
-    __shared__ uint32_t cnt[4];
-    const uint32_t q = blockIdx.x / a.nprobe, p = blockIdx.x % a.nprobe;
-    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
-    const uint32_t cl = (uint32_t)a.probe[(size_t)q * a.probe_stride + p];
-    const uint32_t r0 = a.offsets[cl], r1 = a.offsets[cl + 1];
-    const_as<f32x4> *Q = as_constant(reinterpret_cast<const f32x4 *>(a.q + (size_t)q * a.d4 * 4));
-    WaveTopK tk;
-    tk.init(slots[wid], CW, a.k, nullptr);
-    for (uint32_t t0 = r0 / 64 + wid; t0 * 64 < r1; t0 += 4) {
-        const uint32_t j = t0 * 64 + lane;
-        const bool valid = j >= r0 && j < r1;
-        const float4 *b = a.x + (size_t)t0 * a.d4 * 64 + lane;
-        float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
-        for (uint32_t c = 0; c < a.d4; ++c) {
-            const float4 v = b[(size_t)c * 64];
-            const f32x4 qv = Q[c];
-            l0 = __builtin_fmaf(v.x, qv.x, l0);
-            l1 = __builtin_fmaf(v.y, qv.y, l1);
-            l2 = __builtin_fmaf(v.z, qv.z, l2);
-            l3 = __builtin_fmaf(v.w, qv.w, l3);
-        }
-        const float s = (l0 + l1) + (l2 + l3);
-        tk.offer(valid ? make_key(s, a.orig[j]) : 0, valid);
+// Pair i = q * nprobe + p -> (list, pair) for the sort by list.
+__global__ void pair_prep_kernel(const uint64_t *__restrict__ probe, uint32_t pairs,
+                                 uint32_t *__restrict__ key, uint32_t *__restrict__ val) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < pairs; i += gridDim.x * 256) {
+        key[i] = (uint32_t)probe[i];
+        val[i] = i;
     }
-    tk.compact();
-    if (lane == 0) cnt[wid] = tk.count;
-    __syncthreads();
-    if (wid != 0) return;
-    for (uint32_t w = 1; w < 4; ++w)
-        for (uint32_t i0 = 0; i0 < cnt[w]; i0 += 64) {
-            const uint32_t i = i0 + lane;
-            tk.offer(i < cnt[w] ? slots[w][i] : 0, i < cnt[w]);
-        }
-    tk.compact();
-    uint64_t *dst = a.part + ((size_t)q * a.nprobe + p) * a.k;
-    for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk.count ? slots[0][i] : 0;
 }
 
-// One block per query: its nprobe lists merged into the final top-k.
+// Wavefronts per list: ceil(pairs of the list / qb).
+__global__ void group_count_kernel(const uint32_t *__restrict__ pair_cnt, uint32_t nlist,
+                                   uint32_t qb, uint32_t *__restrict__ grp_cnt) {
+    for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < nlist; c += gridDim.x * 256)
+        grp_cnt[c] = (pair_cnt[c] + qb - 1) / qb;
+}
+
+// Wavefront = (pair group of one list, row chunk of that list).  Lane r of a
+// 64-row tile scores row r against the group's QB queries in the reference
+// dot order (four-lane FMA chains, (l0 + l1) + (l2 + l3)); the queries come
+// through the scalar cache; each query keeps its own wave top-k in LDS and
+// writes its best k keys (original row ids) for this chunk.
+template <int QB, int CW>
+__global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
+    __shared__ uint64_t slots[4][QB][CW];
+    // wave-uniform by construction; readfirstlane lets the compiler keep the
+    // work item, the list and the query pointers in scalar registers
+    const uint32_t lane = __lane_id();
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t item = blockIdx.x * 4 + wid;
+    const uint32_t g = item / a.nchunk, chunk = item % a.nchunk;
+    if (g >= a.grp_off[a.nlist]) return;  // wave-uniform; no block barrier below
+    // the list: grp_off[c] <= g < grp_off[c + 1]
+    uint32_t lo = 0, hi = a.nlist;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.grp_off[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t c = lo;
+    const uint32_t p0 = a.pair_off[c] + (g - a.grp_off[c]) * QB;
+    const uint32_t pn = a.pair_off[c + 1] - p0 < (uint32_t)QB ? a.pair_off[c + 1] - p0 : QB;
+    uint32_t pid[QB];
+    const_as<f32x4> *Q[QB];
+#pragma unroll
+    for (int j = 0; j < QB; ++j) {
+        pid[j] = a.sorted_pair[p0 + ((uint32_t)j < pn ? j : pn - 1)];
+        Q[j] = as_constant(
+            reinterpret_cast<const f32x4 *>(a.q + (size_t)(pid[j] / a.nprobe) * a.d4 * 4));
+    }
+    const uint32_t l0 = a.offsets[c], len = a.offsets[c + 1] - l0;
+    const uint32_t per = (len + a.nchunk - 1) / a.nchunk;
+    const uint32_t r0 = l0 + (chunk * per < len ? chunk * per : len);
+    const uint32_t r1 = l0 + ((chunk + 1) * per < len ? (chunk + 1) * per : len);
+    WaveTopK tk[QB];
+#pragma unroll
+    for (int j = 0; j < QB; ++j) tk[j].init(slots[wid][j], CW, a.k, nullptr);
+    for (uint32_t t = r0 / 64; t * 64 < r1; ++t) {
+        const uint32_t row = t * 64 + lane;
+        const bool valid = row >= r0 && row < r1;
+        const float4 *b = a.x + (size_t)t * a.d4 * 64 + lane;
+        float acc[QB][4];
+#pragma unroll
+        for (int j = 0; j < QB; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.f;
+#pragma unroll 4
+        for (uint32_t c4 = 0; c4 < a.d4; ++c4) {
+            const float4 v = b[(size_t)c4 * 64];
+#pragma unroll
+            for (int j = 0; j < QB; ++j) {
+                const f32x4 qv = Q[j][c4];
+                acc[j][0] = __builtin_fmaf(v.x, qv.x, acc[j][0]);
+                acc[j][1] = __builtin_fmaf(v.y, qv.y, acc[j][1]);
+                acc[j][2] = __builtin_fmaf(v.z, qv.z, acc[j][2]);
+                acc[j][3] = __builtin_fmaf(v.w, qv.w, acc[j][3]);
+            }
+        }
+        const uint32_t id = valid ? a.orig[row] : 0;
+#pragma unroll
+        for (int j = 0; j < QB; ++j) {
+            if ((uint32_t)j >= pn) break;
+            const float sc = (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
+            tk[j].offer(valid ? make_key(sc, id) : 0, valid);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < QB; ++j) {
+        if ((uint32_t)j >= pn) break;
+        tk[j].compact();
+        uint64_t *dst = a.part + ((size_t)pid[j] * a.nchunk + chunk) * a.k;
+        for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk[j].count ? slots[wid][j][i] : 0;
+    }
+}
+
+// One block per query: its nprobe x nchunk partial lists merged into the
+// final top-k and the number found.
 template <int CW>
 __global__ __launch_bounds__(256) void ivf_merge_kernel(IvfScanArgs a) {
     __shared__ uint64_t slots[4][CW];
@@ -505,8 +557,8 @@ __global__ __launch_bounds__(256) void ivf_merge_kernel(IvfScanArgs a) {
     const uint32_t q = blockIdx.x, lane = __lane_id(), wid = threadIdx.x >> 6;
     WaveTopK tk;
     tk.init(slots[wid], CW, a.k, nullptr);
-    const uint64_t *src = a.part + (size_t)q * a.nprobe * a.k;
-    const uint32_t n = a.nprobe * a.k;
+    const uint32_t n = a.nprobe * a.nchunk * a.k;
+    const uint64_t *src = a.part + (size_t)q * n;
     for (uint32_t i0 = wid * 64; i0 < n; i0 += 256) {
         const uint32_t i = i0 + lane;
         const uint64_t key = i < n ? src[i] : 0;
@@ -736,22 +788,117 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void ivf_scan(const IvfScanArgs &a, uint32_t nq, hipStream_t s) {
-    if (nq == 0) return;
-#define NNDEX_IVF_CASE(C)                                                                \
-    case C:                                                                              \
-        hipLaunchKernelGGL(ivf_scan_kernel<C>, dim3(nq * a.nprobe), dim3(256), 0, s, a); \
-        hipLaunchKernelGGL(ivf_merge_kernel<C>, dim3(nq), dim3(256), 0, s, a);           \
-        break;
-    switch (ivf_cw(a.k)) {
-        NNDEX_IVF_CASE(128)
-        NNDEX_IVF_CASE(256)
-        NNDEX_IVF_CASE(512)
-        NNDEX_IVF_CASE(1024)
-    default:
-        throw Error(NNDEX_ERR_UNSUPPORTED, "IVF search supports k <= 960");
+static uint32_t ivf_bits(size_t nlist) {
+    uint32_t bits = 1;
+    while ((1ull << bits) < nlist) ++bits;
+    return bits;
+}
+
+size_t ivf_sort_bytes(size_t pairs, size_t nlist) {
+    size_t bytes = 0;
+    NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (unsigned)pairs, 0,
+                                              ivf_bits(nlist)));
+    return bytes;
+}
+
+void ivf_scan_shape(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k, uint32_t &qb,
+                    uint32_t &nchunk) {
+    const size_t pairs = nq * nprobe;
+    // queries per wavefront: as many as the lists' pair counts use, within
+    // the LDS of 4 waves x QB top-k buffers (QB 8: k <= 192, QB 4: k <= 448)
+    const size_t per_list = (pairs + nlist - 1) / nlist;
+    qb = per_list >= 6 ? 8 : (per_list >= 2 ? 4 : 1);
+    if (qb == 8 && k > 192) qb = 4;
+    if (qb == 4 && k > 448) qb = 1;
+    // row chunks: about 16K wavefronts (16 per SIMD), chunks of >= 64 rows,
+    // and a merge of at most 64K keys per query
+    const size_t waves = (pairs + qb - 1) / qb;
+    size_t nc = (16384 + waves - 1) / waves;
+    const size_t avg = rows / (nlist ? nlist : 1);
+    const size_t by_len = avg / 64 > 1 ? avg / 64 : 1;
+    if (nc > by_len) nc = by_len;
+    const size_t by_merge = 65536 / (nprobe * k) > 1 ? 65536 / (nprobe * k) : 1;
+    if (nc > by_merge) nc = by_merge;
+    if (nc > 64) nc = 64;
+    nchunk = static_cast<uint32_t>(nc < 1 ? 1 : nc);
+}
+
+size_t ivf_scan_scratch_bytes(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k) {
+    uint32_t qb, nchunk;
+    size_t worst = 0;
+    // the part buffer is largest at the most chunks: try every batch size's
+    // shape up to nq (the chunk count only falls as nq grows)
+    for (size_t n = 1; n <= nq; n = n < 16 ? n + 1 : n * 2) {
+        ivf_scan_shape(n, nprobe, nlist, rows, k, qb, nchunk);
+        const size_t b = n * nprobe * nchunk * k * 8;
+        worst = b > worst ? b : worst;
     }
-#undef NNDEX_IVF_CASE
+    ivf_scan_shape(nq, nprobe, nlist, rows, k, qb, nchunk);
+    const size_t b = nq * nprobe * nchunk * k * 8;
+    worst = b > worst ? b : worst;
+    const size_t pairs = nq * nprobe;
+    return worst + pairs * 4 * 4 + (nlist + 1) * 4 * 4 + ivf_sort_bytes(pairs, nlist) + 1024;
+}
+
+void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
+    if (a.nq == 0) return;
+    const uint32_t pairs = a.nq * a.nprobe;
+    uint32_t grid = (pairs + 255) / 256;
+    if (grid > 1024) grid = 1024;
+    hipLaunchKernelGGL(pair_prep_kernel, dim3(grid), dim3(256), 0, s, a.probe, pairs, a.pair_key,
+                       a.pair_val);
+    size_t bytes = a.sort_bytes;
+    NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(a.sort_tmp, bytes, a.pair_key, a.sorted_key,
+                                              a.pair_val, a.sorted_pair, pairs, 0,
+                                              ivf_bits(a.nlist), s));
+    launch_zero_u32(a.pair_cnt, a.nlist, s);
+    hipLaunchKernelGGL(histogram_kernel, dim3(grid < 256 ? grid : 256), dim3(256),
+                       (size_t)a.nlist * sizeof(uint32_t), s, a.sorted_key, pairs, a.nlist,
+                       a.pair_cnt);
+    hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(256), 0, s, a.pair_cnt, a.nlist, a.pair_off);
+    const uint32_t lgrid = (a.nlist + 255) / 256;
+    hipLaunchKernelGGL(group_count_kernel, dim3(lgrid), dim3(256), 0, s, a.pair_cnt, a.nlist,
+                       a.qb, a.grp_cnt);
+    hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(256), 0, s, a.grp_cnt, a.nlist, a.grp_off);
+    // upper bound on the pair groups: ceil(pairs / qb) + one partial group
+    // per list that holds pairs
+    const size_t gmax = (pairs + a.qb - 1) / a.qb + (a.nlist < pairs ? a.nlist : pairs);
+    const size_t waves = gmax * a.nchunk;
+    const uint32_t sgrid = static_cast<uint32_t>((waves + 3) / 4);
+    const uint32_t cw = ivf_cw(a.k);
+#define NNDEX_LIST_CASE(QBV, C)                                                          \
+    if (a.qb == QBV && cw == C) {                                                        \
+        hipLaunchKernelGGL((ivf_list_scan_kernel<QBV, C>), dim3(sgrid), dim3(256), 0, s, a); \
+        launched = true;                                                                 \
+    }
+    bool launched = false;
+    NNDEX_LIST_CASE(8, 128)
+    NNDEX_LIST_CASE(8, 256)
+    NNDEX_LIST_CASE(4, 128)
+    NNDEX_LIST_CASE(4, 256)
+    NNDEX_LIST_CASE(4, 512)
+    NNDEX_LIST_CASE(1, 128)
+    NNDEX_LIST_CASE(1, 256)
+    NNDEX_LIST_CASE(1, 512)
+    NNDEX_LIST_CASE(1, 1024)
+#undef NNDEX_LIST_CASE
+    if (!launched) throw Error(NNDEX_ERR_UNSUPPORTED, "IVF search supports k <= 960");
+    switch (cw) {
+    case 128:
+        hipLaunchKernelGGL(ivf_merge_kernel<128>, dim3(a.nq), dim3(256), 0, s, a);
+        break;
+    case 256:
+        hipLaunchKernelGGL(ivf_merge_kernel<256>, dim3(a.nq), dim3(256), 0, s, a);
+        break;
+    case 512:
+        hipLaunchKernelGGL(ivf_merge_kernel<512>, dim3(a.nq), dim3(256), 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL(ivf_merge_kernel<1024>, dim3(a.nq), dim3(256), 0, s, a);
+        break;
+    }
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 

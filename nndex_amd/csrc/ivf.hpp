@@ -35,21 +35,40 @@ struct IvfIndex {
     ~IvfIndex();
 };
 
+// List-major scan of the probed lists.  The (query, probe) pairs are sorted by
+// list; each wavefront takes up to QB pairs of one list and one row chunk of
+// it, so QB queries share every row load, and writes each pair's best k keys
+// for that chunk; a per-query merge gives the final top-k.
 struct IvfScanArgs {
     const float4 *x;            // reordered rows, tiled
     uint32_t d4;
     const float *q;             // [nq][d4 * 4] normalised padded queries
-    const uint64_t *probe;      // [nq][probe_stride] probed cluster ids
-    uint32_t probe_stride;
-    uint32_t nprobe;
-    const uint32_t *offsets;
-    const uint32_t *orig;
-    uint32_t k;
-    uint64_t *part;             // [nq][nprobe][k] per-list best keys
+    const uint64_t *probe;      // [nq][nprobe] probed cluster ids
+    uint32_t nq, nprobe, nlist, k;
+    uint32_t qb;                // pairs per wavefront (1, 4 or 8)
+    uint32_t nchunk;            // row chunks per list
+    const uint32_t *offsets;    // [nlist + 1] list offsets in the reordered rows
+    const uint32_t *orig;       // [rows] original row of each reordered position
+    // scratch (IvfScratch::reserve sizes them)
+    uint32_t *pair_key, *pair_val, *sorted_key, *sorted_pair;  // [nq * nprobe]
+    uint32_t *pair_cnt, *pair_off;  // [nlist], [nlist + 1]
+    uint32_t *grp_cnt, *grp_off;    // [nlist], [nlist + 1]
+    uint64_t *part;             // [nq * nprobe][nchunk][k]
+    void *sort_tmp;
+    size_t sort_bytes;
     uint64_t *out_idx;          // [nq][k]
     float *out_sim;             // [nq][k]
     uint64_t *out_len;          // [nq]
 };
+
+// Launch shape of the list scan for a batch: pairs per wavefront and row
+// chunks per list (enough wavefronts to fill the device, a bounded merge).
+void ivf_scan_shape(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k, uint32_t &qb,
+                    uint32_t &nchunk);
+// Scratch bytes of the list scan for nq queries (worst case over shapes).
+size_t ivf_scan_scratch_bytes(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k);
+// rocPRIM temporary bytes for sorting `pairs` (list, pair) keys of `nlist` lists.
+size_t ivf_sort_bytes(size_t pairs, size_t nlist);
 
 // Build the index of `rows` tiled rows (pdims padded columns) on stream s.
 // x_max is the largest row norm; above 512 padded dims the assignment runs on
@@ -58,6 +77,6 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                uint32_t pdims, uint32_t nlist, uint32_t iters, double x_max, double x_err,
                uint32_t ncu, hipStream_t s);
 // Scan the probed lists and merge per query (probe ids already on device).
-void ivf_scan(const IvfScanArgs &a, uint32_t nq, hipStream_t s);
+void ivf_scan(const IvfScanArgs &a, hipStream_t s);
 
 }  // namespace nndex
