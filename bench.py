@@ -273,7 +273,7 @@ def main():
                     help="run only the serving form and print its record")
     ap.add_argument("--serve-rows", type=int, default=2_000_000)
     ap.add_argument("--serve-dims", type=int, default=1024)
-    ap.add_argument("--serve-rate", type=float, default=50.0,
+    ap.add_argument("--serve-rate", type=float, default=150.0,
                     help="request arrivals per second (Poisson)")
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-slots", type=int, default=4)

@@ -154,6 +154,17 @@ class NNdex:
         return self._dims
 
     @property
+    def backend_handle(self):
+        """The ``nndex_gpu_index*`` behind a GPU index (for the device entry
+        points and the serving loop of ``nndex_amd.serve``)."""
+        h = _lib.lib().nndex_backend_handle(self._h)
+        return ctypes.c_void_p(h) if h else None
+
+    @property
+    def padded_dims(self) -> int:
+        return (self._dims + 7) // 8 * 8
+
+    @property
     def gpu_info(self) -> dict | None:
         info = _lib.GpuDeviceInfo()
         has = ctypes.c_int32(0)
