@@ -15,6 +15,12 @@ Also reported:
                 (ParallelFold strategy, all threads) on a bounded sample;
   batch         QPS at batch sizes 16 / 256 / 4096 (exact FP32) and the BF16
                 scoring + exact FP32 rerank form at 4096;
+  precision     the precision form: 5,000,000 x 768 anisotropic base (x_i ~
+                N(0,1) * i^-0.5, unit-normalised), 1,024 queries, k=100: the
+                exact search against BF16 and FP8-E4M3 (per-row scale)
+                candidate scoring with 400 over-fetch and exact FP32 rerank —
+                QPS, recall@100, certified fraction, and the returned scores'
+                largest error against FP64 on a query sample;
   serve         the serving form: a resident 2,000,000 x 1,024 index (FP32 +
                 BF16 copies, IVF built), Poisson arrivals at a fixed rate, per
                 request B log-uniform in 1..2048 rows and k in {1, 10, 100},
@@ -244,6 +250,74 @@ def serve_bench(args) -> dict:
     return res
 
 
+def precision_bench(args, k: int = 100) -> dict:
+    """The precision form (see the module docstring)."""
+    import oracle
+
+    L = _lib.lib()
+    rows, dims, nq = args.prec_rows, args.prec_dims, args.prec_queries
+    t0 = time.time()
+    base = synth.anisotropic(rows, dims, seed=42)
+    queries = synth.anisotropic(nq, dims, seed=43)
+    log(f"precision workload {rows}x{dims} generated in {time.time() - t0:.1f}s")
+    h = ctypes.c_void_p()
+    check(L.nndex_gpu_index_new(base.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rows, dims,
+                                0, 0, -1, ctypes.byref(h)))
+    ws = ctypes.c_void_p()
+    check(L.nndex_workspace_new(h, nq, 448, ctypes.byref(ws)))
+    stream = ctypes.c_void_p()
+    check(L.nndex_stream_create(ctypes.byref(stream)))
+    dq = Dev(queries.nbytes).put(queries)
+    di, ds_, fl = Dev(nq * k * 8), Dev(nq * k * 4), Dev(nq)
+
+    def timed(fn, reps=3):
+        fn()
+        check(L.nndex_stream_synchronize(stream))
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        check(L.nndex_stream_synchronize(stream))
+        return (time.perf_counter() - t) / reps
+
+    dt = timed(lambda: check(L.nndex_gpu_index_search_device(h, ws, dq.ptr, nq, k, di.ptr,
+                                                              ds_.ptr, stream)))
+    exact_i = di.get((nq, k), np.uint64)
+    exact_s = ds_.get((nq, k), np.float32)
+    res = {"workload": f"{rows}x{dims} anisotropic (x_i ~ N(0,1) i^-0.5, unit-normalised), "
+                       f"{nq} queries, k={k}",
+           "exact": {"qps": round(nq / dt, 1), "ms_per_batch": round(dt * 1e3, 3)}}
+    log(f"precision exact: {nq / dt:.0f} QPS ({dt * 1e3:.1f} ms)")
+    # FP64 reference on a query sample: largest error of the returned scores
+    sample = list(range(0, nq, max(1, nq // 16)))
+    fp64 = {i: oracle.fp64_scores(base, queries[i], True, threads=min(16, os.cpu_count() or 1))
+            for i in sample}
+    err_exact = max(float(np.max(np.abs(exact_s[i].astype(np.float64) -
+                                         fp64[i][exact_i[i].astype(np.int64)])))
+                    for i in sample)
+    res["exact"]["max_abs_err_vs_fp64"] = err_exact
+    for prec, name in ((1, "bf16"), (2, "fp8")):
+        dt = timed(lambda: check(L.nndex_gpu_index_search_batch_device(
+            h, ws, dq.ptr, nq, k, prec, 4 * k, di.ptr, ds_.ptr, fl.ptr, stream)))
+        gi = di.get((nq, k), np.uint64)
+        gs = ds_.get((nq, k), np.float32)
+        flags = fl.get((nq,), np.uint8)
+        recall = float(np.mean([len(set(gi[i].tolist()) & set(exact_i[i].tolist())) / k
+                                for i in range(nq)]))
+        err = max(float(np.max(np.abs(gs[i].astype(np.float64) -
+                                      fp64[i][gi[i].astype(np.int64)]))) for i in sample)
+        res[name] = {"qps": round(nq / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
+                     f"recall_at_{k}": round(recall, 5), "overfetch": 4 * k,
+                     "certified": round(1.0 - float(flags.mean()), 4),
+                     "max_abs_err_vs_fp64": err,
+                     "speedup_vs_exact": round(res["exact"]["ms_per_batch"] / (dt * 1e3), 3)}
+        log(f"precision {name}: {nq / dt:.0f} QPS ({dt * 1e3:.1f} ms), recall@{k} {recall:.5f}, "
+            f"certified {1 - flags.mean():.3f}, max |err| vs fp64 {err:.2e}")
+    L.nndex_stream_destroy(stream)
+    L.nndex_workspace_free(ws)
+    L.nndex_gpu_index_free(h)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +342,12 @@ def main():
     ap.add_argument("--ivf-nlist", type=int, default=4096)
     ap.add_argument("--ivf-iters", type=int, default=20)
     ap.add_argument("--ivf-queries", type=int, default=1024)
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="run only the precision form and print its record")
+    ap.add_argument("--prec-rows", type=int, default=5_000_000)
+    ap.add_argument("--prec-dims", type=int, default=768)
+    ap.add_argument("--prec-queries", type=int, default=1024)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving form and print its record")
@@ -288,6 +368,9 @@ def main():
     L = _lib.lib()
     if args.ivf_only:
         print(json.dumps({"ivf": ivf_bench(args)}), flush=True)
+        return
+    if args.precision_only:
+        print(json.dumps({"precision": precision_bench(args)}), flush=True)
         return
     if args.serve_only:
         print(json.dumps({"serve": serve_bench(args)}), flush=True)
@@ -480,6 +563,7 @@ def main():
     L.nndex_workspace_free(ws)
     L.nndex_gpu_index_free(h)
     del base
+    precision = None if args.no_precision else precision_bench(args)
     ivf = None if args.no_ivf else ivf_bench(args)
     serve = None if args.no_serve else serve_bench(args)
 
@@ -510,6 +594,7 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
         "autotune": tune.strip().splitlines()[-1] if tune else None,
         "batch": batch,
+        "precision": precision,
         "ivf": ivf,
         "serve": serve,
     }

@@ -149,12 +149,18 @@ nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspa
                                            size_t k, uint64_t *device_indices,
                                            float *device_similarities, void *stream);
 
-/* Candidate-scoring precision of the batched path. */
-typedef enum nndex_precision { NNDEX_PRECISION_F32 = 0, NNDEX_PRECISION_BF16 = 1 } nndex_precision;
+/* Candidate-scoring precision of the batched path.  FP8 is OCP E4M3 with a
+ * per-row scale max|x_r| / 448 (and a per-query scale likewise). */
+typedef enum nndex_precision {
+    NNDEX_PRECISION_F32 = 0,
+    NNDEX_PRECISION_BF16 = 1,
+    NNDEX_PRECISION_FP8 = 2
+} nndex_precision;
 
 /* Batched search: MFMA candidate scoring in `precision` (FP32 accumulation),
  * exact FP32 rerank of the best `overfetch` candidates (0 = default: k + 16
- * or 1.5k for F32, 4k for BF16), and a per-query certificate written to
+ * or 1.5k for F32, max(4k, k + 64) for BF16 and FP8; at most 447), and a
+ * per-query certificate written to
  * device_flags (0 = result proven identical to the exact FP32 search, 1 =
  * not proven: re-run that query with nndex_gpu_index_search_device).
  * Fully asynchronous on `stream`.  No reference counterpart (the reference

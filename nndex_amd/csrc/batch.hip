@@ -29,6 +29,7 @@ namespace nndex {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
 // Element traits: one 16-byte unit per lane per MFMA step group.
@@ -50,6 +51,8 @@ struct F32E {
     static __device__ inline float total(const v4f (&c)[NACC], int v) {
         return (c[0][v] + c[1][v]) + (c[2][v] + c[3][v]);
     }
+    static constexpr bool SCALED = false, PAIR = false;
+    static __device__ inline void mma2(uint4, uint4, uint4, uint4, v4f (&)[NACC]) {}
 };
 struct BF16E {
     // one 16x16x32 bf16 MFMA over the unit's eight bf16 values
@@ -59,7 +62,105 @@ struct BF16E {
                                                        __builtin_bit_cast(v8bf, b), c[0], 0, 0, 0);
     }
     static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
+    static constexpr bool SCALED = false, PAIR = false;
+    static __device__ inline void mma2(uint4, uint4, uint4, uint4, v4f (&)[NACC]) {}
 };
+struct FP8E {
+    // one 16x16x128 f8f6f4 MFMA (both operands OCP E4M3, unit block scales)
+    // per two 16-byte steps: lane group g supplies the 32 codes of its units
+    // 4s+g and 4(s+1)+g, rows and queries alike, so the K order is the same
+    // on both sides; the score is rescaled by the row and query scales in the
+    // epilogue
+    static constexpr int NACC = 1;
+    static constexpr bool SCALED = true, PAIR = true;
+    static __device__ inline void mma(uint4, uint4, v4f (&)[NACC]) {}
+    static __device__ inline void mma2(uint4 a0, uint4 a1, uint4 b0, uint4 b1, v4f (&c)[NACC]) {
+        const v8i A = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                       (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+        const v8i B = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
+                       (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+        c[0] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(A, B, c[0], 0, 0, 0, 127, 0, 127);
+    }
+    static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
+};
+
+// ---------------------------------------------------------------------------
+// FP8-E4M3 (OCP: bias 7, max 448, no infinities) encode with round to nearest
+// even, saturating at +-448, and exact decode.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t fp8_e4m3(float v) {
+    const uint32_t sign = (__builtin_bit_cast(uint32_t, v) >> 24) & 0x80u;
+    const float a = __builtin_fabsf(v);
+    uint32_t code;
+    if (!(a < 464.0f)) {
+        code = 0x7Eu;
+    } else if (a < 0x1p-6f) {
+        code = (uint32_t)__builtin_rintf(a * 512.0f);  // subnormal steps of 2^-9 (8 = 2^-6)
+    } else {
+        const uint32_t b = __builtin_bit_cast(uint32_t, a);
+        const uint32_t e = (b >> 23) - 120u;  // 127 - 7
+        const uint32_t keep = (b >> 20) & 7u, rem = b & 0xFFFFFu;
+        uint32_t c = (e << 3) | keep;
+        if (rem > 0x80000u || (rem == 0x80000u && (keep & 1u))) c += 1;
+        code = c > 0x7Eu ? 0x7Eu : c;
+    }
+    return sign | code;
+}
+__device__ inline float fp8_e4m3_value(uint32_t c) {
+    const uint32_t e = (c >> 3) & 15u, m = c & 7u;
+    const float v = e == 0 ? (float)m * 0x1p-9f
+                           : __builtin_bit_cast(float, ((e + 120u) << 23) | (m << 20));
+    return (c & 0x80u) ? -v : v;
+}
+
+// Tiled FP32 -> tiled FP8 with a per-row scale; one thread per row (lanes of a
+// wave read and write 64 consecutive rows of a tile: both coalesced).
+__global__ __launch_bounds__(256) void to_fp8_kernel(const float4 *__restrict__ x,
+                                                     uint32_t rows_padded, uint32_t d4,
+                                                     uint4 *__restrict__ y,
+                                                     float *__restrict__ scale,
+                                                     unsigned long long *err_sq) {
+    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    double ss = 0.0;
+    if (row < rows_padded) {
+        const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
+        float mx = 0.f;
+        for (uint32_t c = 0; c < d4; ++c) {
+            const float4 v = b[(size_t)c * 64];
+            mx = __builtin_fmaxf(mx, __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(v.x),
+                                                                    __builtin_fabsf(v.y)),
+                                                     __builtin_fmaxf(__builtin_fabsf(v.z),
+                                                                    __builtin_fabsf(v.w))));
+        }
+        const float sc = mx > 0.f ? mx / 448.0f : 0.f;
+        const uint32_t d16 = (d4 + 3) / 4;
+        uint4 *o = y + (size_t)(row / 64) * d16 * 64 + row % 64;
+        for (uint32_t u = 0; u < d16; ++u) {
+            uint32_t w[4];
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t c = 4 * u + j;
+                const float4 v = c < d4 ? b[(size_t)c * 64] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float e[4] = {v.x, v.y, v.z, v.w};
+                uint32_t word = 0;
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t code = sc > 0.f ? fp8_e4m3(e[i] / sc) : 0u;
+                    word |= code << (8 * i);
+                    const double d = (double)fp8_e4m3_value(code) * sc - (double)e[i];
+                    ss += d * d;
+                }
+                w[j] = word;
+            }
+            o[(size_t)u * 64] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        scale[row] = sc;
+    }
+    unsigned long long bits = __builtin_bit_cast(unsigned long long, ss);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long other = __shfl_xor(bits, off);
+        bits = other > bits ? other : bits;
+    }
+    if (__lane_id() == 0) atomicMax(err_sq, bits);
+}
 
 // ---------------------------------------------------------------------------
 // BF16 copy of the tiled FP32 matrix (round to nearest even), same tiling in
@@ -136,60 +237,102 @@ __global__ __launch_bounds__(256) void max_norm_kernel(const float4 *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Query preparation: low-precision copy (row-major, 16-byte units) and the
-// per-query error bound delta of |approximate - exact| over any row:
-//   exact side   gamma_n |x| |q|   (four-lane FMA chain, n = padded dims)
-//   approx side  gamma_n |x~| |q~| (MFMA accumulation)
-//   rounding     |x~| |q~ - q| + u_e |x| |q|   (u_e = 2^-8 for bf16, 0 for f32)
-// with |x| <= xmax (largest row norm) and |x~| <= (1 + u_e) xmax.
+// Query preparation: low-precision copy (row-major, 16-byte units; FP8 with a
+// per-query scale s_q = max|q| / 448) and the per-query error bound delta of
+// |approximate - exact| over any row, with x~, q~ the dequantised copies
+// (x~ = x, q~ = q for FP32):
+//   exact side   gamma_n |x| |q|                 (four-lane FMA chain)
+//   approx side  gamma_n |x~| |q~| (+ 3u |x~| |q~| for the FP8 rescaling)
+//   rounding     |x~| |q~ - q| + |x~ - x| |q|
+// with |x| <= xmax (largest row norm), |x~ - x| <= xe (largest row rounding
+// error, measured when the copy was made) and |x~| <= xmax + xe.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restrict__ q,
                                                            uint32_t nq, uint32_t pdims,
-                                                           int bf16, uint4 *__restrict__ qlow,
+                                                           int elem, uint4 *__restrict__ qlow,
+                                                           float *__restrict__ q_scale,
                                                            const unsigned long long *xmax_sq,
                                                            const unsigned long long *xerr_sq,
                                                            float *__restrict__ delta) {
     const uint32_t qi = blockIdx.x;
     if (qi >= nq) return;
     const float *src = q + (size_t)qi * pdims;
-    double nq2 = 0, nl2 = 0, nd2 = 0;
-    for (uint32_t i = threadIdx.x; i < pdims; i += 256) {
-        double v = src[i];
-        double lv = bf16 ? (double)(float)(__bf16)src[i] : v;
-        nq2 += v * v;
-        nl2 += lv * lv;
-        nd2 += (lv - v) * (lv - v);
+    __shared__ double red[3][256];
+    __shared__ float smax[256];
+    float sq = 1.f;  // FP8 query scale
+    if (elem == kElemFP8) {
+        float mx = 0.f;
+        for (uint32_t i = threadIdx.x; i < pdims; i += 256) mx = __builtin_fmaxf(mx, __builtin_fabsf(src[i]));
+        smax[threadIdx.x] = mx;
+        __syncthreads();
+        for (uint32_t st = 128; st > 0; st >>= 1) {
+            if (threadIdx.x < st)
+                smax[threadIdx.x] = __builtin_fmaxf(smax[threadIdx.x], smax[threadIdx.x + st]);
+            __syncthreads();
+        }
+        sq = smax[0] > 0.f ? smax[0] / 448.0f : 0.f;
+        if (threadIdx.x == 0) q_scale[qi] = sq;
     }
-    if (bf16) {
-        uint32_t d8 = pdims / 8;
-        for (uint32_t c = threadIdx.x; c < d8; c += 256) {
-            const float *s = src + c * 8;
-            qlow[(size_t)qi * d8 + c] = make_uint4(pack_bf16x2(s[0], s[1]), pack_bf16x2(s[2], s[3]),
-                                                   pack_bf16x2(s[4], s[5]), pack_bf16x2(s[6], s[7]));
+    double nq2 = 0, nl2 = 0, nd2 = 0;
+    if (elem == kElemFP8) {
+        const uint32_t d16 = (pdims + 15) / 16;
+        for (uint32_t u = threadIdx.x; u < d16; u += 256) {
+            uint32_t w[4] = {0, 0, 0, 0};
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t i = u * 16 + j;
+                const float v = i < pdims ? src[i] : 0.f;
+                const uint32_t code = sq > 0.f ? fp8_e4m3(v / sq) : 0u;
+                w[j / 4] |= code << (8 * (j % 4));
+                const double lv = (double)fp8_e4m3_value(code) * sq;
+                nq2 += (double)v * v;
+                nl2 += lv * lv;
+                nd2 += (lv - v) * (lv - v);
+            }
+            qlow[(size_t)qi * d16 + u] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < pdims; i += 256) {
+            const double v = src[i];
+            const double lv = elem == kElemBF16 ? (double)(float)(__bf16)src[i] : v;
+            nq2 += v * v;
+            nl2 += lv * lv;
+            nd2 += (lv - v) * (lv - v);
+        }
+        if (elem == kElemBF16) {
+            const uint32_t d8 = pdims / 8;
+            for (uint32_t c = threadIdx.x; c < d8; c += 256) {
+                const float *s = src + c * 8;
+                qlow[(size_t)qi * d8 + c] =
+                    make_uint4(pack_bf16x2(s[0], s[1]), pack_bf16x2(s[2], s[3]),
+                               pack_bf16x2(s[4], s[5]), pack_bf16x2(s[6], s[7]));
+            }
         }
     }
-    __shared__ double red[3][256];
     red[0][threadIdx.x] = nq2;
     red[1][threadIdx.x] = nl2;
     red[2][threadIdx.x] = nd2;
     __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + s];
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + st];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double xm = sqrt(__builtin_bit_cast(double, *xmax_sq));
-        double ue = bf16 ? 0x1p-8 : 0.0;
-        double u = 0x1p-24, n = pdims;
-        double gamma = n * u / (1.0 - n * u);
-        double qn = sqrt(red[0][0]), ql = sqrt(red[1][0]), qd = sqrt(red[2][0]);
-        double xl = xm * (1.0 + ue);
-        // |sum (x~ - x) q| <= |x~ - x| |q|, with the largest row rounding error
-        // measured when the BF16 copy was made (never above u_e |x|)
-        double xe = bf16 ? sqrt(__builtin_bit_cast(double, *xerr_sq)) : 0.0;
-        if (xe > ue * xm) xe = ue * xm;
-        double d = gamma * xm * qn + gamma * xl * ql + xl * qd + xe * qn;
+        const double xm = sqrt(__builtin_bit_cast(double, *xmax_sq));
+        const double u = 0x1p-24, n = pdims;
+        const double gamma = n * u / (1.0 - n * u);
+        const double qn = sqrt(red[0][0]), ql = sqrt(red[1][0]), qd = sqrt(red[2][0]);
+        double xe = 0.0;
+        if (elem == kElemBF16) {
+            // the BF16 rounding error is also at most 2^-8 |x|
+            xe = sqrt(__builtin_bit_cast(double, *xerr_sq));
+            if (xe > 0x1p-8 * xm) xe = 0x1p-8 * xm;
+        } else if (elem == kElemFP8) {
+            xe = sqrt(__builtin_bit_cast(double, *xerr_sq));
+        }
+        const double xl = xm + xe;
+        const double ga = elem == kElemFP8 ? gamma + 3.0 * u : gamma;
+        const double d = gamma * xm * qn + ga * xl * ql + xl * qd + xe * qn;
         delta[qi] = (float)(d * (1.0 + 1e-6)) + 1e-30f;
     }
 }
@@ -279,6 +422,15 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) best[m][v] = second[m][v] = 0;
         }
+        float rs[MT][4];  // FP8 row scales of the lane's rows
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                rs[m][v] = 1.f;
+                if constexpr (E::SCALED)
+                    rs[m][v] = a.row_scale[bt * block_rows + wid * RW + m * 16 + 4 * g + v];
+            }
         __syncthreads();  // previous block tile done with both buffers
         fetch(0);
         commit(0);
@@ -300,37 +452,53 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                     for (int j = 0; j < E::NACC; ++j) acc[m][t][j] = v4f{0.f, 0.f, 0.f, 0.f};
             // one-step software pipeline over LDS reads; the scheduling
-            // barriers keep the compiler from hoisting every step's reads
-            uint4 qc[NT], qn[NT];
+            // barriers keep the compiler from hoisting every step's reads.
+            // An MMA covers SW steps (SW = 2 for the K=128 FP8 form).
+            constexpr int SW = E::PAIR ? 2 : 1;
+            uint4 qc[SW][NT], qn[SW][NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) qc[t] = src[g * kStride + t * 16 + l16];
+            for (int i = 0; i < SW; ++i)
 #pragma unroll
-            for (int s = 0; s < MAXS; ++s) {
-                if (s + 1 < MAXS) {
+                for (int t = 0; t < NT; ++t) qc[i][t] = src[(4 * i + g) * kStride + t * 16 + l16];
 #pragma unroll
-                    for (int t = 0; t < NT; ++t)
-                        qn[t] = src[(4 * (s + 1) + g) * kStride + t * 16 + l16];
+            for (int s = 0; s < MAXS; s += SW) {
+                if (s + SW < MAXS) {
+#pragma unroll
+                    for (int i = 0; i < SW; ++i)
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            qn[i][t] = src[(4 * (s + SW + i) + g) * kStride + t * 16 + l16];
                 }
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
 #pragma unroll
-                    for (int m = 0; m < MT; ++m) E::mma(xr[m][s], qc[t], acc[m][t]);
+                    for (int m = 0; m < MT; ++m) {
+                        if constexpr (E::PAIR)
+                            E::mma2(xr[m][s], xr[m][s + 1], qc[0][t], qc[1][t], acc[m][t]);
+                        else
+                            E::mma(xr[m][s], qc[0][t], acc[m][t]);
+                    }
 #pragma unroll
-                for (int t = 0; t < NT; ++t) qc[t] = qn[t];
+                for (int i = 0; i < SW; ++i)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) qc[i][t] = qn[i][t];
                 __builtin_amdgcn_sched_barrier(0);
             }
             // ---- epilogue: lane holds rows 4*g+v of each tile, query 16t+l16 ----
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const uint32_t qi = ch * QC + t * 16 + l16;
+                float qs = 1.f;
+                if constexpr (E::SCALED) qs = qi < a.nq ? a.q_scale[qi] : 0.f;
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
+                        float sc = E::total(acc[m][t], v);
+                        if constexpr (E::SCALED) sc = sc * (qs * rs[m][v]);
                         if constexpr (AM) {
-                            const uint64_t key =
-                                qi < a.nq ? make_key(E::total(acc[m][t], v), qi) : 0;
+                            const uint64_t key = qi < a.nq ? make_key(sc, qi) : 0;
                             if (key > best[m][v]) {
                                 second[m][v] = best[m][v];
                                 best[m][v] = key;
@@ -338,14 +506,14 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
                                 second[m][v] = key;
                             }
                         } else if (a.mode == 0) {
-                            const uint64_t key = make_key(E::total(acc[m][t], v), row);
+                            const uint64_t key = make_key(sc, row);
                             const bool pass = qi < a.nq && row < a.rows && key >= th_cur[t];
                             if (__ballot(pass) != 0 && pass) {
                                 uint32_t pos = atomicAdd(a.cand_count + qi, 1u);
                                 if (pos < a.capq) a.cand[(size_t)qi * a.capq + pos] = key;
                             }
                         } else if (qi < a.nq && row < a.rows) {
-                            a.out[(size_t)qi * a.ldo + row] = E::total(acc[m][t], v);
+                            a.out[(size_t)qi * a.ldo + row] = sc;
                         }
                     }
                 }
@@ -669,11 +837,19 @@ void launch_max_norm(const float4 *x, uint32_t rows, uint32_t d4, unsigned long 
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, bool bf16, uint4 *qlow,
-                         const unsigned long long *xmax_sq, const unsigned long long *xerr_sq,
-                         float *delta, hipStream_t s) {
-    hipLaunchKernelGGL(prep_queries_kernel, dim3(nq), dim3(256), 0, s, q, nq, pdims, bf16 ? 1 : 0,
-                       qlow, xmax_sq, xerr_sq, delta);
+void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, int elem, uint4 *qlow,
+                         float *q_scale, const unsigned long long *xmax_sq,
+                         const unsigned long long *xerr_sq, float *delta, hipStream_t s) {
+    hipLaunchKernelGGL(prep_queries_kernel, dim3(nq), dim3(256), 0, s, q, nq, pdims, elem, qlow,
+                       q_scale, xmax_sq, xerr_sq, delta);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, float *scale,
+                   unsigned long long *err_sq, hipStream_t s) {
+    const uint32_t grid = (uint32_t)((rows_padded + 255) / 256);
+    hipLaunchKernelGGL(to_fp8_kernel, dim3(grid), dim3(256), 0, s, x, (uint32_t)rows_padded, d4,
+                       y, scale, err_sq);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
@@ -695,35 +871,42 @@ static uint32_t mfma_steps(uint32_t dch) {
     return 0;
 }
 
-uint32_t mfma_block_rows(bool bf16, uint32_t dch) {
-    return bf16 && mfma_steps(dch) <= 8 ? 8 * 32 : 8 * 16;
+uint32_t mfma_block_rows(int elem, uint32_t dch) {
+    const uint32_t st = mfma_steps(dch);
+    if (elem == kElemFP8) return st <= 6 ? 8 * 32 : 8 * 16;
+    return elem == kElemBF16 && st <= 8 ? 8 * 32 : 8 * 16;
 }
 
-void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream_t s) {
+void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s) {
+    const bool bf16 = elem == kElemBF16, fp8 = elem == kElemFP8;
     const uint32_t st = mfma_steps(a.dch);
     if (st == 0)
-        throw Error(NNDEX_ERR_UNSUPPORTED, "padded dims above 512 (f32) / 1024 (bf16) are not "
-                                           "supported by the MFMA scan");
-#define NNDEX_MFMA_CASE(STEPS, F32_NT, BF_RW, BF_NT)                                   \
+        throw Error(NNDEX_ERR_UNSUPPORTED, "padded dims above 512 (f32) / 1024 (bf16) / 2048 "
+                                           "(fp8) are not supported by the MFMA scan");
+#define NNDEX_MFMA_CASE(STEPS, F32_NT, BF_RW, BF_NT, F8_RW, F8_NT)                    \
     case STEPS:                                                                        \
-        if (a.mode == 2 && !bf16)                                                      \
+        if (a.mode == 2 && !bf16 && !fp8)                                              \
             return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT, true>(a, grid, s);        \
-        if (a.mode == 2)                                                               \
+        if (a.mode == 2 && bf16)                                                       \
             return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT, true>(a, grid, s);     \
+        if (a.mode == 2) break;                                                        \
+        if (fp8)                                                                       \
+            return launch_mfma_t<FP8E, 8, F8_RW, STEPS, F8_NT>(a, grid, s);            \
         if (bf16)                                                                      \
             return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT>(a, grid, s);           \
         return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT>(a, grid, s);
     switch (st) {
-        // (steps, F32 query tiles, BF16 rows per wave, BF16 query tiles):
-        // the largest tiles that fit 256 VGPRs without spills
-        NNDEX_MFMA_CASE(2, 4, 32, 4)
-        NNDEX_MFMA_CASE(4, 4, 32, 4)
-        NNDEX_MFMA_CASE(6, 2, 32, 2)
-        NNDEX_MFMA_CASE(8, 2, 32, 2)
-        NNDEX_MFMA_CASE(12, 2, 16, 4)
-        NNDEX_MFMA_CASE(16, 2, 16, 2)
-        NNDEX_MFMA_CASE(24, 1, 16, 1)
-        NNDEX_MFMA_CASE(32, 1, 16, 1)
+        // (steps, F32 query tiles, BF16 rows per wave, BF16 query tiles, FP8
+        // rows per wave, FP8 query tiles): the largest tiles that fit 256
+        // VGPRs without spills
+        NNDEX_MFMA_CASE(2, 4, 32, 4, 32, 4)
+        NNDEX_MFMA_CASE(4, 4, 32, 4, 32, 2)
+        NNDEX_MFMA_CASE(6, 2, 32, 2, 32, 2)
+        NNDEX_MFMA_CASE(8, 2, 32, 2, 16, 2)
+        NNDEX_MFMA_CASE(12, 2, 16, 4, 16, 1)
+        NNDEX_MFMA_CASE(16, 2, 16, 2, 16, 1)
+        NNDEX_MFMA_CASE(24, 1, 16, 1, 16, 1)
+        NNDEX_MFMA_CASE(32, 1, 16, 1, 16, 1)
     default:
         break;
     }

@@ -7,13 +7,18 @@
 
 namespace nndex {
 
+// Element type of the MFMA scan (the candidate-scoring precision).
+enum ElemKind : int { kElemF32 = 0, kElemBF16 = 1, kElemFP8 = 2 };
+
 struct MfmaScanArgs {
-    const uint4 *x;            // tiled matrix in 16-byte units (f32: 4 values, bf16: 8)
+    const uint4 *x;            // tiled matrix in 16-byte units (f32: 4 values, bf16: 8, fp8: 16)
     uint32_t rows;             // logical rows
     uint32_t rows_padded;      // rows rounded up to 64 (allocated tiles)
     uint32_t dch;              // 16-byte units per row
     const uint4 *q;            // queries, row-major, q_stride units per query
     uint32_t q_stride;
+    const float *row_scale;    // fp8: per-row scale [rows_padded]
+    const float *q_scale;      // fp8: per-query scale [nq]
     uint32_t nq;
     uint32_t mode;             // 0: append candidates, 1: write scores, 2: row argmax
     const unsigned long long *thresh;  // [nq] key thresholds (mode 0)
@@ -48,15 +53,20 @@ struct RerankArgs {
 };
 
 void launch_to_bf16(const float4 *x, size_t rows_padded, uint32_t d8, uint4 *y, hipStream_t s);
+// FP8-E4M3 (OCP) copy with a per-row scale s_r = max|x_r| / 448: tiled units of
+// 16 codes (d16 = ceil(d4 / 4) units per row), scales [rows_padded], and the
+// largest per-row norm of |s_r x~_r - x_r| (f64 bits, atomic max into *err_sq).
+void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, float *scale,
+                   unsigned long long *err_sq, hipStream_t s);
 void launch_max_norm(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
                      hipStream_t s);
-void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, bool bf16, uint4 *qlow,
-                         const unsigned long long *xmax_sq, const unsigned long long *xerr_sq,
-                         float *delta, hipStream_t s);
+void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, int elem, uint4 *qlow,
+                         float *q_scale, const unsigned long long *xmax_sq,
+                         const unsigned long long *xerr_sq, float *delta, hipStream_t s);
 void launch_bf16_error(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
                        hipStream_t s);
-uint32_t mfma_block_rows(bool bf16, uint32_t dch);
-void launch_mfma_scan(const MfmaScanArgs &a, bool bf16, uint32_t grid, hipStream_t s);
+uint32_t mfma_block_rows(int elem, uint32_t dch);
+void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s);
 void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
                              uint32_t kp, const float *delta, unsigned long long *thresh,
                              hipStream_t s);

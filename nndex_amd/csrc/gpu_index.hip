@@ -45,7 +45,7 @@ PinnedBuffer::~PinnedBuffer() {
 }
 
 Workspace::~Workspace() {
-    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta,
+    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta, (void *)qscale,
                     (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
                     (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
                     (void *)xflags, (void *)ivf_pair_u32, (void *)ivf_list_u32, ivf_sort_tmp})
@@ -64,6 +64,7 @@ void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t cap
         regrow(qlow, nq * pdims / 8);
         regrow(delta, nq);
         regrow(thresh, nq);
+        regrow(qscale, nq);
         regrow(cand_count, nq);
         regrow(flags, nq);
         b_queries = nq;
@@ -168,9 +169,13 @@ GpuIndex::~GpuIndex() {
     if (xbf16_) (void)hipFree(xbf16_);
     if (xmax_sq_) (void)hipFree(xmax_sq_);
     if (xerr_sq_) (void)hipFree(xerr_sq_);
+    for (void *p : {(void *)xfp8_, (void *)xscale_, (void *)xerr8_sq_})
+        if (p) (void)hipFree(p);
     for (auto &smp : samples_) {
         if (smp.f32) (void)hipFree(smp.f32);
         if (smp.bf16) (void)hipFree(smp.bf16);
+        for (void *p : {(void *)smp.fp8, (void *)smp.fp8_scale, (void *)smp.fp8_err})
+            if (p) (void)hipFree(p);
     }
 }
 
@@ -222,7 +227,20 @@ void GpuIndex::ensure_bf16(hipStream_t s) {
     NNDEX_HIP_CHECK(hipStreamSynchronize(s));
 }
 
-const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s) {
+void GpuIndex::ensure_fp8(hipStream_t s) {
+    std::lock_guard<std::mutex> l(lazy_mu_);
+    if (xfp8_) return;
+    const size_t prows = padded_rows(rows_), d16 = (d4_ + 3) / 4;
+    NNDEX_HIP_CHECK(hipMalloc(&xfp8_, prows * d16 * sizeof(uint4)));
+    NNDEX_HIP_CHECK(hipMalloc(&xscale_, prows * sizeof(float)));
+    NNDEX_HIP_CHECK(hipMalloc(&xerr8_sq_, sizeof(unsigned long long)));
+    NNDEX_HIP_CHECK(hipMemsetAsync(xerr8_sq_, 0, sizeof(unsigned long long), s));
+    launch_to_fp8(x_, prows, static_cast<uint32_t>(d4_), xfp8_, xscale_, xerr8_sq_, s);
+    // complete before the lock is released: another stream may use it next
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+const RowSample &GpuIndex::sample_for(uint32_t stride, Precision p, hipStream_t s) {
     std::lock_guard<std::mutex> l(lazy_mu_);
     RowSample *hit = nullptr;
     for (auto &smp : samples_)
@@ -237,7 +255,17 @@ const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s)
         launch_gather_rows(x_, static_cast<uint32_t>(d4_), stride, hit->rows, hit->f32, s);
         NNDEX_HIP_CHECK(hipStreamSynchronize(s));
     }
-    if (bf16 && !hit->bf16) {
+    if (p == Precision::FP8 && !hit->fp8) {
+        const size_t prows = padded_rows(hit->rows), d16 = (d4_ + 3) / 4;
+        NNDEX_HIP_CHECK(hipMalloc(&hit->fp8, prows * d16 * sizeof(uint4)));
+        NNDEX_HIP_CHECK(hipMalloc(&hit->fp8_scale, prows * sizeof(float)));
+        NNDEX_HIP_CHECK(hipMalloc(&hit->fp8_err, sizeof(unsigned long long)));
+        NNDEX_HIP_CHECK(hipMemsetAsync(hit->fp8_err, 0, sizeof(unsigned long long), s));
+        launch_to_fp8(hit->f32, prows, static_cast<uint32_t>(d4_), hit->fp8, hit->fp8_scale,
+                      hit->fp8_err, s);
+        NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    if (p == Precision::BF16 && !hit->bf16) {
         const size_t prows = padded_rows(hit->rows);
         NNDEX_HIP_CHECK(hipMalloc(&hit->bf16, prows * (pdims_ / 8) * sizeof(uint4)));
         launch_to_bf16(hit->f32, prows, static_cast<uint32_t>(pdims_ / 8), hit->bf16, s);
@@ -246,10 +274,12 @@ const RowSample &GpuIndex::sample_for(uint32_t stride, bool bf16, hipStream_t s)
     return *hit;
 }
 
-bool GpuIndex::mfma_supported(Precision p) const {
-    const size_t units = p == Precision::F32 ? pdims_ / 4 : pdims_ / 8;
-    return (units + 3) / 4 <= 32;
+// 16-byte units per row of the MFMA scan's element type.
+static size_t mfma_units(size_t pdims, Precision p) {
+    return p == Precision::F32 ? pdims / 4 : (p == Precision::BF16 ? pdims / 8 : (pdims + 15) / 16);
 }
+
+bool GpuIndex::mfma_supported(Precision p) const { return (mfma_units(pdims_, p) + 3) / 4 <= 32; }
 
 size_t GpuIndex::default_overfetch(size_t k, Precision p) {
     size_t kp = p == Precision::F32 ? k + (k / 2 > 16 ? k / 2 : 16)
@@ -264,23 +294,29 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
     if (!mfma_supported(p))
         throw Error(NNDEX_ERR_UNSUPPORTED, "dims too large for the MFMA batch path");
     if (k == 0 || k > rows_) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+    const bool fp8 = p == Precision::FP8;
+    const int elem = static_cast<int>(p);
     const BatchPlan plan = plan_batch(k, p, kp);
     kp = plan.kp;
     if (bf16) ensure_bf16(s);
-    const RowSample &smp = sample_for(plan.stride, bf16, s);
+    if (fp8) ensure_fp8(s);
+    const RowSample &smp = sample_for(plan.stride, p, s);
     const size_t capq = plan.capq;
     const size_t group = plan.group;
-    const uint32_t dch = static_cast<uint32_t>(bf16 ? pdims_ / 8 : pdims_ / 4);
+    const uint32_t dch = static_cast<uint32_t>(mfma_units(pdims_, p));
     for (size_t q0 = 0; q0 < nq; q0 += group) {
         const size_t n = nq - q0 < group ? nq - q0 : group;
         ws.reserve_batch(n, pdims_, smp.rows, capq);
         const float *qf = dq + q0 * pdims_;
-        launch_prep_queries(qf, static_cast<uint32_t>(n), static_cast<uint32_t>(pdims_), bf16,
-                            ws.qlow, xmax_sq_, bf16 ? xerr_sq_ + 1 : xmax_sq_, ws.delta, s);
-        const uint4 *qu = bf16 ? ws.qlow : reinterpret_cast<const uint4 *>(qf);
+        launch_prep_queries(qf, static_cast<uint32_t>(n), static_cast<uint32_t>(pdims_), elem,
+                            ws.qlow, ws.qscale, xmax_sq_,
+                            bf16 ? xerr_sq_ + 1 : (fp8 ? xerr8_sq_ : xmax_sq_), ws.delta, s);
+        const uint4 *qu = p != Precision::F32 ? ws.qlow : reinterpret_cast<const uint4 *>(qf);
         // stage 1: sample scores and thresholds
         MfmaScanArgs a{};
-        a.x = bf16 ? smp.bf16 : reinterpret_cast<const uint4 *>(smp.f32);
+        a.x = bf16 ? smp.bf16 : (fp8 ? smp.fp8 : reinterpret_cast<const uint4 *>(smp.f32));
+        a.row_scale = smp.fp8_scale;
+        a.q_scale = ws.qscale;
         a.rows = smp.rows;
         a.rows_padded = static_cast<uint32_t>(padded_rows(smp.rows));
         a.dch = dch;
@@ -290,18 +326,19 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.mode = 1;
         a.out = ws.scores;
         a.ldo = smp.rows;
-        const uint32_t br = mfma_block_rows(bf16, dch);
+        const uint32_t br = mfma_block_rows(elem, dch);
         auto grid_for = [&](uint32_t rp) {
             uint32_t blocks = (rp + br - 1) / br;
             uint32_t cap = ncu_;  // one 8-wave block per CU (register-bound)
             return blocks < cap ? blocks : cap;
         };
-        launch_mfma_scan(a, bf16, grid_for(a.rows_padded), s);
+        launch_mfma_scan(a, elem, grid_for(a.rows_padded), s);
         launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, smp.rows,
                                 static_cast<uint32_t>(kp), ws.delta, ws.thresh, s);
         // stage 2: filter scan over every row
         launch_zero_u32(ws.cand_count, n, s);
-        a.x = bf16 ? xbf16_ : reinterpret_cast<const uint4 *>(x_);
+        a.x = bf16 ? xbf16_ : (fp8 ? xfp8_ : reinterpret_cast<const uint4 *>(x_));
+        a.row_scale = xscale_;
         a.rows = static_cast<uint32_t>(rows_);
         a.rows_padded = static_cast<uint32_t>(padded_rows(rows_));
         a.mode = 0;
@@ -309,7 +346,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.cand = ws.cand;
         a.cand_count = ws.cand_count;
         a.capq = static_cast<uint32_t>(capq);
-        launch_mfma_scan(a, bf16, grid_for(a.rows_padded), s);
+        launch_mfma_scan(a, elem, grid_for(a.rows_padded), s);
         // stages 3-4: exact rerank and certificate
         RerankArgs r{};
         r.x = x_;
@@ -351,7 +388,7 @@ BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
 
 size_t BatchPlan::workspace_bytes(size_t nq, size_t pdims) const {
     const size_t n = nq < group ? nq : group;
-    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 1) + n * rows_s * 4 + n * capq * 8;
+    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 4 + 1) + n * rows_s * 4 + n * capq * 8;
 }
 
 bool GpuIndex::exact_batch_precision(Precision &p) const {
@@ -386,7 +423,7 @@ void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hi
         size_t rows_s = 0, capq = 0, group = 0;
         for (size_t kk = 1; kk <= k && kk <= 400 && kk <= rows_; ++kk) {
             const BatchPlan plan = plan_batch(kk, p, 0);
-            sample_for(plan.stride, bf16, s);
+            sample_for(plan.stride, p, s);
             rows_s = plan.rows_s > rows_s ? plan.rows_s : rows_s;
             capq = plan.capq > capq ? plan.capq : capq;
             group = plan.group;
@@ -1045,7 +1082,8 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
         if (ws->owner != index->impl.get())
             throw Error(NNDEX_ERR_INVALID_ARGUMENT, "workspace belongs to another index");
         if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
-        if (precision != NNDEX_PRECISION_F32 && precision != NNDEX_PRECISION_BF16)
+        if (precision != NNDEX_PRECISION_F32 && precision != NNDEX_PRECISION_BF16 &&
+            precision != NNDEX_PRECISION_FP8)
             throw Error(NNDEX_ERR_INVALID_ARGUMENT, "unknown precision");
         index->impl->search_batch_mfma(*ws->impl, device_queries, query_rows, k,
                                        static_cast<Precision>(precision), overfetch,

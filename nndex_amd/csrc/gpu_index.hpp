@@ -25,7 +25,7 @@ struct Neighbor {
 };
 
 // Candidate-scoring precision of the batched path.
-enum class Precision : int { F32 = 0, BF16 = 1 };
+enum class Precision : int { F32 = 0, BF16 = 1, FP8 = 2 };
 
 // Scratch of the fused scan: block lists, thresholds and the ticket; plus the
 // batched path's buffers, grown on demand.
@@ -45,6 +45,7 @@ struct Workspace {
     float *delta = nullptr;
     float *scores = nullptr;
     unsigned long long *thresh = nullptr;
+    float *qscale = nullptr;
     uint64_t *cand = nullptr;
     uint32_t *cand_count = nullptr;
     uint8_t *flags = nullptr;
@@ -84,6 +85,9 @@ struct RowSample {
     uint32_t stride = 0, rows = 0;
     float4 *f32 = nullptr;
     uint4 *bf16 = nullptr;
+    uint4 *fp8 = nullptr;
+    float *fp8_scale = nullptr;
+    unsigned long long *fp8_err = nullptr;
 };
 
 struct DeviceBuffer {
@@ -213,8 +217,12 @@ class GpuIndex {
     unsigned long long *xerr_sq_ = nullptr;  // [0] unused, [1] BF16 rounding error
     std::mutex lazy_mu_;
     std::deque<RowSample> samples_;  // references stay valid as it grows
-    const RowSample &sample_for(uint32_t stride, bool bf16, hipStream_t s);
+    const RowSample &sample_for(uint32_t stride, Precision p, hipStream_t s);
     void ensure_bf16(hipStream_t s);
+    void ensure_fp8(hipStream_t s);
+    uint4 *xfp8_ = nullptr;                  // FP8-E4M3 copy, tiled units of 16
+    float *xscale_ = nullptr;                // its per-row scales [rows_padded]
+    unsigned long long *xerr8_sq_ = nullptr; // largest row rounding-error norm^2
     hipStream_t stream_ = nullptr;
     nndex_gpu_device_info info_{};
     std::mutex mu_;

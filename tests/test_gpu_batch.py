@@ -108,6 +108,41 @@ def test_bf16_rerank_recall_and_certificate(orc, dims, k):
         assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
 
 
+@pytest.mark.parametrize("dims,k,anisotropic", [(384, 10, False), (128, 100, False),
+                                               (768, 100, True), (1000, 10, False)])
+def test_fp8_rerank_recall(orc, dims, k, anisotropic):
+    """FP8-E4M3 scoring (per-row and per-query scales) with max(4k, k+64)
+    over-fetch and exact FP32 rerank: every returned similarity is the exact
+    FP32 score of its row (bit for bit), results are sorted, certified
+    queries equal the exact search, and recall@k against exact is high."""
+    _case("fp8", dims=dims, k=k, anisotropic=anisotropic)
+    rows = 60_000
+    if anisotropic:
+        base = synth.anisotropic(rows, dims, seed=3)
+        queries = synth.anisotropic(256, dims, seed=4)
+    else:
+        base, queries = synth.mixture_workload(rows=rows, dims=dims, n_queries=256)
+    pdims = (dims + 7) // 8 * 8
+    if pdims != dims:
+        base = np.pad(base, ((0, 0), (0, pdims - dims)))
+        queries = np.pad(queries, ((0, 0), (0, pdims - dims)))
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 256, max(4 * k, k + 64))
+    gi, gs, fl = batch_device(b, ds, queries, k, 2)
+    ei, es = exact_device(b, ds, queries, k)
+    cert = fl == 0
+    recall = np.mean([len(set(gi[i].tolist()) & set(ei[i].tolist())) / k for i in range(256)])
+    print(f"certified {cert.mean():.3f} recall@{k} {recall:.4f}", flush=True)
+    assert recall >= 0.97
+    for i in np.nonzero(cert)[0]:
+        assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
+    for i in range(0, 256, 17):
+        assert np.all(np.diff(gs[i].astype(np.float64)) <= 0)
+        for j in range(0, k, max(1, k // 7)):
+            row = int(gi[i, j])
+            assert gs[i, j] == np.float32(orc.dot_product(base[row], queries[i])), (i, j)
+
+
 def test_batch_position_independence_mfma():
     _case("mfma_position")
     rng = np.random.default_rng(9)
