@@ -162,7 +162,10 @@ typedef enum nndex_precision {
  * or 1.5k for F32, max(4k, k + 64) for BF16 and FP8; at most 447), and a
  * per-query certificate written to
  * device_flags (0 = result proven identical to the exact FP32 search, 1 =
- * not proven: re-run that query with nndex_gpu_index_search_device).
+ * not proven: re-run that query with nndex_gpu_index_search_device).  BF16
+ * and F32 rescore every candidate above the threshold for queries the first
+ * rerank does not prove; FP8 returns the exact rerank of the over-fetched
+ * candidates as is (its certificate is reported, rarely reached).
  * Fully asynchronous on `stream`.  No reference counterpart (the reference
  * scores batches in FP32 only and reads every score back, src/gpu.rs:640-785). */
 nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_workspace *ws,
@@ -171,6 +174,24 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
                                                  uint64_t *device_indices,
                                                  float *device_similarities,
                                                  uint8_t *device_flags, void *stream);
+
+/* HIP graph of one batched search (no reference counterpart).  Records the
+ * launches nndex_gpu_index_search_batch_device makes for exactly these
+ * arguments (buffers, query_rows, k, precision, overfetch) on `stream` (not
+ * the null stream) into an instantiated graph; every lazy copy and workspace
+ * buffer is created first, so a replay allocates nothing.  Replay with
+ * nndex_graph_launch after writing new queries into device_queries: the
+ * outputs are those of a fresh batched search.  The workspace belongs to the
+ * graph while it exists (no other call may use it concurrently). */
+typedef struct nndex_graph nndex_graph;
+nndex_status nndex_gpu_index_capture_batch(nndex_gpu_index *index, nndex_workspace *ws,
+                                           const float *device_queries, size_t query_rows,
+                                           size_t k, int32_t precision, size_t overfetch,
+                                           uint64_t *device_indices, float *device_similarities,
+                                           uint8_t *device_flags, void *stream,
+                                           nndex_graph **out);
+nndex_status nndex_graph_launch(nndex_graph *graph, void *stream);
+void nndex_graph_free(nndex_graph *graph);
 
 /* Autotuner (no reference counterpart): times the fused scan over candidate
  * launch shapes (64-row tiles per wavefront) on this index and keeps the

@@ -179,6 +179,10 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_double)]),
     "nndex_server_get_stats": (_ST, [_P, ctypes.POINTER(ServerStats)]),
     "nndex_server_reset_stats": (_ST, [_P]),
+    "nndex_gpu_index_capture_batch": (_ST, [_P, _P, _P, _SZ, _SZ, _I32, _SZ, _P, _P, _P, _P,
+                                            ctypes.POINTER(_P)]),
+    "nndex_graph_launch": (_ST, [_P, _P]),
+    "nndex_graph_free": (None, [_P]),
     "nndex_device_copy": (_ST, [_P, _P, _SZ, _P]),
     "nndex_new": (_ST, [_FP, _SZ, _SZ, _SZ, ctypes.POINTER(IndexOptions), ctypes.POINTER(_P)]),
     "nndex_free": (None, [_P]),

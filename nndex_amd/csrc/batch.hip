@@ -857,8 +857,13 @@ template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
 static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     const size_t lds = 2ull * MAXS * 4 * (16 * NT + 1) * sizeof(uint4);
     auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM>;
-    NNDEX_HIP_CHECK(hipFuncSetAttribute((const void *)fn,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // once per instantiation (also keeps the call out of graph captures)
+    static bool attr_set = false;
+    if (!attr_set) {
+        NNDEX_HIP_CHECK(hipFuncSetAttribute((const void *)fn,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        attr_set = true;
+    }
     hipLaunchKernelGGL(fn, dim3(grid), dim3(NW * 64), lds, s, a);
     NNDEX_HIP_CHECK(hipGetLastError());
 }

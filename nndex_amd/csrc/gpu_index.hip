@@ -363,7 +363,9 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         r.out_sim = d_sim + q0 * k;
         r.flags = d_flags + q0;
         launch_rerank(r, static_cast<uint32_t>(n), s);
-        launch_rerank_full(r, static_cast<uint32_t>(n), s);
+        // FP8's error bound rarely certifies: its result is the exact rerank
+        // of the over-fetched candidates, and the flag reports it unproven
+        if (!fp8) launch_rerank_full(r, static_cast<uint32_t>(n), s);
     }
 }
 
@@ -1091,6 +1093,71 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
                                        static_cast<hipStream_t>(stream));
     });
 }
+
+}  // extern "C"
+
+struct nndex_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~nndex_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+extern "C" {
+
+nndex_status nndex_gpu_index_capture_batch(nndex_gpu_index *index, nndex_workspace *ws,
+                                           const float *device_queries, size_t query_rows,
+                                           size_t k, int32_t precision, size_t overfetch,
+                                           uint64_t *device_indices, float *device_similarities,
+                                           uint8_t *device_flags, void *stream,
+                                           nndex_graph **out) {
+    return guard([&] {
+        if (!index || !ws || !device_queries || !device_indices || !device_similarities ||
+            !device_flags || !out)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (!stream) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "capture needs a non-null stream");
+        if (ws->owner != index->impl.get())
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "workspace belongs to another index");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        if (precision != NNDEX_PRECISION_F32 && precision != NNDEX_PRECISION_BF16 &&
+            precision != NNDEX_PRECISION_FP8)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "unknown precision");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        GpuIndex &g = *index->impl;
+        NNDEX_HIP_CHECK(hipSetDevice(g.device()));
+        // one eager run creates every lazy copy and workspace buffer
+        g.search_batch_mfma(*ws->impl, device_queries, query_rows, k,
+                            static_cast<Precision>(precision), overfetch, device_indices,
+                            device_similarities, device_flags, s);
+        NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+        std::unique_ptr<nndex_graph> gr(new nndex_graph());
+        NNDEX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            g.search_batch_mfma(*ws->impl, device_queries, query_rows, k,
+                                static_cast<Precision>(precision), overfetch, device_indices,
+                                device_similarities, device_flags, s);
+        } catch (...) {
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(s, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        NNDEX_HIP_CHECK(hipStreamEndCapture(s, &gr->graph));
+        NNDEX_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
+        *out = gr.release();
+    });
+}
+
+nndex_status nndex_graph_launch(nndex_graph *graph, void *stream) {
+    return guard([&] {
+        if (!graph) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        NNDEX_HIP_CHECK(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+    });
+}
+
+void nndex_graph_free(nndex_graph *graph) { delete graph; }
 
 nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char *report,
                                      size_t report_len) {
