@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void normalize_rowmajor_kernel(
 // `part`, and the last block to finish (ticket) merges all block lists,
 // writes the outputs and resets the ticket and thresholds for the next launch.
 // ---------------------------------------------------------------------------
-template <int QB, int CW>
+template <int QB, int CW, int U, bool NT>
 __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     __shared__ uint64_t slots[4][QB][CW];
     __shared__ uint32_t wave_count[4][QB];
@@ -200,19 +200,23 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     const float4 *__restrict__ X = a.x;
     const_as<f32x4> *Q = as_constant(reinterpret_cast<const f32x4 *>(a.q));
 
+    // Per-wave thresholds only: the device-wide one costs more than it
+    // prunes here, since reading it back waits for every outstanding row
+    // load of the wave (vector memory counters complete in order).  Blocks
+    // publish all their keys (the theta slots stay 0) to the final merge.
     WaveTopK tk[QB];
 #pragma unroll
-    for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, a.theta + q);
+    for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, nullptr);
 
-    constexpr int U = 8;  // float4 chunks per load group
-    const bool nt = a.flags & kScanNonTemporal;
+    // U float4 chunks per load group (two groups in flight); NT selects
+    // streaming (non-temporal) loads at compile time so the loop is straight
     auto load_group = [&](float4(&buf)[U], const float4 *p, uint32_t cbase) {
-        if (nt) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) buf[u] = load_nt(p + (size_t)(cbase + u) * 64);
-        } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) buf[u] = p[(size_t)(cbase + u) * 64];
+        for (int u = 0; u < U; ++u) {
+            if constexpr (NT)
+                buf[u] = load_nt(p + (size_t)(cbase + u) * 64);
+            else
+                buf[u] = p[(size_t)(cbase + u) * 64];
         }
     };
     auto fma_group = [&](float(&acc)[QB][4], const float4(&buf)[U], uint32_t cbase) {
@@ -257,10 +261,10 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     if (d4 % (2 * U) == 0) {
         // Two load groups in flight: group g+1 (or the next tile's first
         // group) is issued before group g is consumed, so the wave keeps
-        // 8-16 KiB of reads outstanding across tile boundaries.
+        // U-2U KiB of reads outstanding across tile boundaries.
         float4 A[U], B[U];
         const float4 *p = X + (size_t)t * d4 * 64 + lane;
-        if (t < tend) load_group(A, p, 0);
+        if (t < tend) load_group(A, p, 0);  // (wave-uniform)
         while (t < tend) {
             const uint32_t tn = t + tstride;
             const float4 *pn = X + (size_t)tn * d4 * 64 + lane;
@@ -270,10 +274,12 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
             for (uint32_t c = 0; c < d4; c += 2 * U) {
                 load_group(B, p, c + U);
                 fma_group(acc, A, c);
-                if (c + 2 * U < d4)
-                    load_group(A, p, c + 2 * U);
-                else if (tn < tend)
-                    load_group(A, pn, 0);
+                // next group A: this tile's next chunks, else the next tile's
+                // first (after the last tile: this tile's first again, an
+                // in-bounds re-read that keeps the loop free of branches)
+                const float4 *na = c + 2 * U < d4 ? p + (size_t)(c + 2 * U) * 64
+                                                  : (tn < tend ? pn : p);
+                load_group(A, na, 0);
                 fma_group(acc, B, c + U);
             }
             finish_tile(acc, t);
@@ -288,7 +294,11 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
             const float4 *xt = X + (size_t)t * d4 * 64 + lane;
 #pragma unroll 8
             for (uint32_t c = 0; c < d4; ++c) {
-                float4 v = load_nt(xt + (size_t)c * 64);
+                float4 v;
+                if constexpr (NT)
+                    v = load_nt(xt + (size_t)c * 64);
+                else
+                    v = xt[(size_t)c * 64];
 #pragma unroll
                 for (int q = 0; q < QB; ++q) {
                     const uint32_t qq = (uint32_t)q < a.nvalid ? (uint32_t)q : a.nvalid - 1;
@@ -347,25 +357,35 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     if (!is_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    // Final merge of gridDim.x block lists per query, keys >= final threshold.
+    // Final merge of gridDim.x block lists per query, keys >= final threshold:
+    // the block counts go to LDS in one round trip, then every (block, slot)
+    // key is loaded in independent batches of 8 per lane before it is offered.
+    __shared__ uint32_t bcnt[2048];  // grids are capped at 8 blocks per CU
+    const uint32_t nb = gridDim.x;
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-        unsigned long long theta =
+        const unsigned long long theta =
             __hip_atomic_load(a.theta + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tk[q].init(slots[wid][q], CW, a.k, nullptr);
-        for (uint32_t b0 = wid * 64; b0 < gridDim.x; b0 += 256) {
-            const uint32_t b = b0 + lane;
-            const uint64_t *src = a.part + ((size_t)q * gridDim.x + b) * a.k;
-            const uint32_t cnt = b < gridDim.x ? a.part_count[(size_t)q * gridDim.x + b] : 0;
-            uint32_t maxc = cnt;
-            for (int off = 32; off > 0; off >>= 1) {
-                uint32_t o = __shfl_xor(maxc, off);
-                maxc = o > maxc ? o : maxc;
+        __syncthreads();  // bcnt reuse across queries
+        for (uint32_t b = threadIdx.x; b < nb && b < 2048; b += 256)
+            bcnt[b] = a.part_count[(size_t)q * nb + b];
+        __syncthreads();
+        const uint64_t *src = a.part + (size_t)q * nb * a.k;
+        const uint32_t total = nb * a.k;
+        constexpr int KB = 8;
+        for (uint32_t j0 = wid * 64 * KB; j0 < total; j0 += 256 * KB) {
+            uint64_t key[KB];
+            bool ok[KB];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const uint32_t j = j0 + u * 64 + lane;
+                const uint32_t b = j / a.k, i = j % a.k;
+                ok[u] = j < total && i < bcnt[b];
+                key[u] = ok[u] ? src[j] : 0;
             }
-            for (uint32_t i = 0; i < maxc; ++i) {
-                const uint64_t key = i < cnt ? src[i] : 0;
-                tk[q].offer(key, i < cnt && key >= theta);
-            }
+#pragma unroll
+            for (int u = 0; u < KB; ++u) tk[q].offer(key[u], ok[u] && key[u] >= theta);
         }
         tk[q].compact();
         if (lane == 0) wave_count[wid][q] = tk[q].count;
@@ -421,9 +441,30 @@ void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, u
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
+template <int QB, int CW, bool NT>
+static void launch_scan_nt(const ScanArgs &a, uint32_t grid, hipStream_t s) {
+    // single-query scans may use larger load groups (more bytes in flight
+    // per wave); the group must divide half a row
+    const uint32_t code = (a.flags >> kScanGroupShift) & 3u;
+    if constexpr (QB == 1) {
+        if (code == 2 && a.d4 % 32 == 0) {
+            hipLaunchKernelGGL((exact_scan_kernel<QB, CW, 16, NT>), dim3(grid), dim3(256), 0, s, a);
+            return;
+        }
+        if (code == 1 && a.d4 % 24 == 0) {
+            hipLaunchKernelGGL((exact_scan_kernel<QB, CW, 12, NT>), dim3(grid), dim3(256), 0, s, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((exact_scan_kernel<QB, CW, 8, NT>), dim3(grid), dim3(256), 0, s, a);
+}
+
 template <int QB, int CW>
 static void launch_scan_t(const ScanArgs &a, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((exact_scan_kernel<QB, CW>), dim3(grid), dim3(256), 0, s, a);
+    if (a.flags & kScanNonTemporal)
+        launch_scan_nt<QB, CW, true>(a, grid, s);
+    else
+        launch_scan_nt<QB, CW, false>(a, grid, s);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 

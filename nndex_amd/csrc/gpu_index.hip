@@ -552,7 +552,8 @@ uint32_t GpuIndex::scan_grid() const {
     const uint32_t tiles = static_cast<uint32_t>(padded_rows(rows_) / kTileRows);
     const uint32_t m = scan_tiles_per_wave();
     uint32_t g = (tiles + 4 * m - 1) / (4 * m);
-    const uint32_t cap = ncu_ * 8;
+    // at most 8 blocks per CU and 2048 blocks (the final merge's LDS table)
+    const uint32_t cap = ncu_ * 8 < 2048 ? ncu_ * 8 : 2048;
     if (g > cap) g = cap;
     return g ? g : 1;
 }
@@ -574,8 +575,13 @@ std::string GpuIndex::autotune(int reps) {
     std::string report;
     float best_ms = 1e30f;
     uint32_t best_m = 0, best_f = kScanNonTemporal;
-    for (uint32_t f : {kScanNonTemporal, 0u, kScanNonTemporal | kScanContiguous, kScanContiguous}) {
-        for (uint32_t m : {2u, 3u, 4u, 6u, 8u, 12u, 16u, 20u, 24u, 32u, 48u}) {
+    std::vector<uint32_t> flag_set;
+    for (uint32_t grp : {0u, 1u, 2u})
+        for (uint32_t f : {kScanNonTemporal, 0u, kScanNonTemporal | kScanContiguous,
+                           kScanContiguous})
+            flag_set.push_back(f | (grp << kScanGroupShift));
+    for (uint32_t f : flag_set) {
+        for (uint32_t m : {2u, 3u, 4u, 6u, 8u, 12u, 14u, 16u, 18u, 20u, 24u, 32u, 48u}) {
             if ((tiles + 4 * m - 1) / (4 * m) > ncu_ * 8) continue;
             scan_tpw_ = m;
             scan_flags_ = f;

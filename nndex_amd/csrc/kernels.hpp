@@ -10,6 +10,8 @@ namespace nndex {
 // Launch-shape flags of the fused exact scan.
 constexpr uint32_t kScanNonTemporal = 1;  // streaming (non-temporal) loads
 constexpr uint32_t kScanContiguous = 2;   // contiguous tile runs per wave
+// bits 2-3: load group of the single-query scan, 0 = 8, 1 = 12, 2 = 16 float4
+constexpr uint32_t kScanGroupShift = 2;
 
 // Arguments of the fused exact scan (exact.hip).
 struct ScanArgs {

@@ -41,13 +41,18 @@ struct WaveTopK {
         global_thresh = g;
     }
 
+    // Bitonic sort (descending) of the first n slots, n the smallest power of
+    // two holding the count (slots past the count are cleared to key 0).
     __device__ void sort_desc() {
         const uint32_t lane = __lane_id();
-        for (uint32_t i = count + lane; i < cap; i += 64) buf[i] = 0;
+        uint32_t n = 2;
+        while (n < count) n <<= 1;
+        if (n > cap) n = cap;
+        for (uint32_t i = count + lane; i < n; i += 64) buf[i] = 0;
         wave_lds_sync();
-        for (uint32_t size = 2; size <= cap; size <<= 1) {
+        for (uint32_t size = 2; size <= n; size <<= 1) {
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t p = lane; p < cap / 2; p += 64) {
+                for (uint32_t p = lane; p < n / 2; p += 64) {
                     uint32_t lo = 2 * p - (p & (stride - 1));
                     uint32_t hi = lo + stride;
                     bool desc = (lo & size) == 0;
