@@ -271,17 +271,19 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
             float acc[QB][4];
 #pragma unroll
             for (int q = 0; q < QB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.f;
-            for (uint32_t c = 0; c < d4; c += 2 * U) {
+            // the chunk loop without its last pair keeps a straight body; the
+            // last pair prefetches the next tile's first group (if any)
+            uint32_t c = 0;
+            for (; c + 2 * U < d4; c += 2 * U) {
                 load_group(B, p, c + U);
                 fma_group(acc, A, c);
-                // next group A: this tile's next chunks, else the next tile's
-                // first (after the last tile: this tile's first again, an
-                // in-bounds re-read that keeps the loop free of branches)
-                const float4 *na = c + 2 * U < d4 ? p + (size_t)(c + 2 * U) * 64
-                                                  : (tn < tend ? pn : p);
-                load_group(A, na, 0);
+                load_group(A, p, c + 2 * U);
                 fma_group(acc, B, c + U);
             }
+            load_group(B, p, c + U);
+            fma_group(acc, A, c);
+            if (tn < tend) load_group(A, pn, 0);  // (wave-uniform)
+            fma_group(acc, B, c + U);
             finish_tile(acc, t);
             t = tn;
             p = pn;
