@@ -201,7 +201,8 @@ nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char
                                       size_t report_len);
 /* Set the fused scan's launch shape: 64-row tiles per wavefront (0 =
  * default) and flags (bit 0: non-temporal loads, bit 1: contiguous tile runs
- * per wavefront instead of interleaved tiles). */
+ * per wavefront instead of interleaved tiles, bits 2-3: single-query load
+ * group of 8 / 12 / 16 float4 per lane, two groups in flight). */
 nndex_status nndex_gpu_index_set_launch(nndex_gpu_index *index, uint32_t scan_tiles_per_wave,
                                         uint32_t scan_flags);
 
@@ -260,7 +261,9 @@ nndex_status nndex_gpu_index_search_ivf_device(nndex_gpu_index *index, nndex_wor
  * max_batch_queries x max_k) takes the oldest pending request plus every
  * later one of the same kind that fits and runs them as one batch at their
  * largest k, so work arriving while batches run is admitted as executors free
- * and overlaps on the device.  Admission by HBM footprint: the number of
+ * and overlaps on the device; with use_graphs, exact batches replay captured
+ * HIP graphs (one per rows/k bucket and executor).  Admission by HBM
+ * footprint: the number of
  * executors is the largest whose fixed footprint fits hbm_budget_bytes;
  * nothing is allocated while serving.  A request's results equal those of the
  * same request served alone. */
@@ -270,6 +273,9 @@ typedef struct nndex_server_options {
     uint32_t max_k;             /* largest k served; 0 = 100 */
     uint32_t ivf_nprobe_max;    /* 0 = exact requests only; else IVF up to this nprobe */
     uint64_t hbm_budget_bytes;  /* device bytes for all executors; 0 = 16 GiB */
+    uint32_t use_graphs;        /* 1 = exact batches above 8 rows replay HIP
+                                   graphs captured per (rows, k) power-of-two
+                                   bucket; 0 = eager launches */
 } nndex_server_options;
 
 typedef struct nndex_server_stats {
@@ -282,6 +288,7 @@ typedef struct nndex_server_stats {
     double p50_ms, p99_ms, max_ms;  /* submit -> results ready, per request */
     double span_s;              /* first submit -> last completion */
     double queries_per_s, requests_per_s;
+    uint64_t graph_batches;     /* batches served by a captured graph */
 } nndex_server_stats;
 
 typedef struct nndex_server nndex_server;

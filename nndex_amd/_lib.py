@@ -115,6 +115,7 @@ class ServerOptions(ctypes.Structure):
         ("max_k", ctypes.c_uint32),
         ("ivf_nprobe_max", ctypes.c_uint32),
         ("hbm_budget_bytes", ctypes.c_uint64),
+        ("use_graphs", ctypes.c_uint32),
     ]
 
 
@@ -134,6 +135,7 @@ class ServerStats(ctypes.Structure):
         ("span_s", ctypes.c_double),
         ("queries_per_s", ctypes.c_double),
         ("requests_per_s", ctypes.c_double),
+        ("graph_batches", ctypes.c_uint64),
     ]
 
     def to_dict(self) -> dict:

@@ -60,6 +60,7 @@ static void regrow(T *&p, size_t n) {
 }
 
 void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq) {
+    if (nq > b_queries || nq * rows_s > b_scores || nq * capq > b_cand) ++generation;
     if (nq > b_queries) {
         regrow(qlow, nq * pdims / 8);
         regrow(delta, nq);
@@ -369,6 +370,37 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
     }
 }
 
+BatchGraph::~BatchGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+void BatchGraph::launch(hipStream_t s) const { NNDEX_HIP_CHECK(hipGraphLaunch(exec, s)); }
+
+std::unique_ptr<BatchGraph> GpuIndex::capture_batch(Workspace &ws, const float *dq, size_t nq,
+                                                    size_t k, Precision p, size_t kp,
+                                                    uint64_t *d_idx, float *d_sim,
+                                                    uint8_t *d_flags, hipStream_t s) {
+    if (!s) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "capture needs a non-null stream");
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    search_batch_mfma(ws, dq, nq, k, p, kp, d_idx, d_sim, d_flags, s);
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+    std::unique_ptr<BatchGraph> g(new BatchGraph());
+    NNDEX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        search_batch_mfma(ws, dq, nq, k, p, kp, d_idx, d_sim, d_flags, s);
+    } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(s, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+    }
+    NNDEX_HIP_CHECK(hipStreamEndCapture(s, &g->graph));
+    NNDEX_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->generation = ws.generation;
+    return g;
+}
+
 BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
     BatchPlan b;
     if (kp == 0) kp = default_overfetch(k, p);
@@ -407,6 +439,7 @@ bool GpuIndex::exact_batch_precision(Precision &p) const {
 
 void Workspace::reserve_xflags(size_t nq) {
     if (nq <= xflags_n) return;
+    ++generation;
     if (xflags) NNDEX_HIP_CHECK(hipFree(xflags));
     xflags = nullptr;
     NNDEX_HIP_CHECK(hipMalloc(&xflags, nq + 256));
@@ -1103,12 +1136,7 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
 }  // extern "C"
 
 struct nndex_graph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    ~nndex_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-    }
+    std::unique_ptr<BatchGraph> impl;
 };
 
 extern "C" {
@@ -1130,28 +1158,11 @@ nndex_status nndex_gpu_index_capture_batch(nndex_gpu_index *index, nndex_workspa
         if (precision != NNDEX_PRECISION_F32 && precision != NNDEX_PRECISION_BF16 &&
             precision != NNDEX_PRECISION_FP8)
             throw Error(NNDEX_ERR_INVALID_ARGUMENT, "unknown precision");
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        GpuIndex &g = *index->impl;
-        NNDEX_HIP_CHECK(hipSetDevice(g.device()));
-        // one eager run creates every lazy copy and workspace buffer
-        g.search_batch_mfma(*ws->impl, device_queries, query_rows, k,
-                            static_cast<Precision>(precision), overfetch, device_indices,
-                            device_similarities, device_flags, s);
-        NNDEX_HIP_CHECK(hipStreamSynchronize(s));
         std::unique_ptr<nndex_graph> gr(new nndex_graph());
-        NNDEX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            g.search_batch_mfma(*ws->impl, device_queries, query_rows, k,
-                                static_cast<Precision>(precision), overfetch, device_indices,
-                                device_similarities, device_flags, s);
-        } catch (...) {
-            hipGraph_t dead = nullptr;
-            (void)hipStreamEndCapture(s, &dead);
-            if (dead) (void)hipGraphDestroy(dead);
-            throw;
-        }
-        NNDEX_HIP_CHECK(hipStreamEndCapture(s, &gr->graph));
-        NNDEX_HIP_CHECK(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0));
+        gr->impl = index->impl->capture_batch(
+            *ws->impl, device_queries, query_rows, k, static_cast<Precision>(precision),
+            overfetch, device_indices, device_similarities, device_flags,
+            static_cast<hipStream_t>(stream));
         *out = gr.release();
     });
 }
@@ -1159,7 +1170,7 @@ nndex_status nndex_gpu_index_capture_batch(nndex_gpu_index *index, nndex_workspa
 nndex_status nndex_graph_launch(nndex_graph *graph, void *stream) {
     return guard([&] {
         if (!graph) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
-        NNDEX_HIP_CHECK(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+        graph->impl->launch(static_cast<hipStream_t>(stream));
     });
 }
 

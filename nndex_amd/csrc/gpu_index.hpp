@@ -31,6 +31,7 @@ enum class Precision : int { F32 = 0, BF16 = 1, FP8 = 2 };
 // batched path's buffers, grown on demand.
 struct Workspace {
     int device = 0;
+    uint64_t generation = 0;  // bumped whenever a buffer is reallocated
     size_t max_queries = 0;
     size_t max_k = 0;
     uint32_t grid = 0;
@@ -78,6 +79,15 @@ struct BatchPlan {
     size_t group = 2048;
     // device bytes of the workspace buffers for nq queries
     size_t workspace_bytes(size_t nq, size_t pdims) const;
+};
+
+// An instantiated HIP graph of one batched search (fixed buffers and shape).
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;  // of the workspace it was captured with
+    void launch(hipStream_t s) const;
+    ~BatchGraph();
 };
 
 // Strided row sample used by the batched path's threshold stage.
@@ -147,6 +157,12 @@ class GpuIndex {
     void search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size_t k, Precision p,
                            size_t kp, uint64_t *d_idx, float *d_sim, uint8_t *d_flags,
                            hipStream_t s);
+    // Capture search_batch_mfma with these arguments into a graph (one eager
+    // run first creates every lazy buffer, so replays allocate nothing).
+    std::unique_ptr<BatchGraph> capture_batch(Workspace &ws, const float *dq, size_t nq,
+                                              size_t k, Precision p, size_t kp,
+                                              uint64_t *d_idx, float *d_sim, uint8_t *d_flags,
+                                              hipStream_t s);
     // Exact search of any batch: the fused scan for small batches; for larger
     // ones the F32 MFMA path, then the scan for any uncertified query (this
     // synchronises `s` once to read the flags).

@@ -15,6 +15,11 @@
 //     total, so the first k of a top-k' list are the top-k).
 //   * Work that arrives while batches run is admitted as soon as an executor
 //     frees, so batches of different kinds overlap on the device.
+//   * With use_graphs, exact batches above 8 rows replay a HIP graph captured
+//     per executor for
+//     (rows rounded up to a power of two, k rounded up to a power of two):
+//     padding rows repeat the last row, each row keeps its own first k
+//     results, and rows the certificate leaves open run the exact fused scan.
 //   * Admission by HBM footprint: an executor's device memory is fixed at
 //     start (workspace for max_batch_queries x max_k of each enabled mode plus
 //     query/result buffers); the number of executors is the largest that fits
@@ -29,6 +34,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -65,8 +71,10 @@ struct Segment {
 struct Executor {
     hipStream_t stream = nullptr;
     std::unique_ptr<Workspace> ws;
-    DeviceBuffer dq, didx, dsim, dlen;
-    PinnedBuffer hq, hidx, hsim, hlen;
+    DeviceBuffer dq, didx, dsim, dlen, dflags;
+    PinnedBuffer hq, hidx, hsim, hlen, hflags;
+    // captured exact batches by (rows bucket, k bucket), made on first use
+    std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<BatchGraph>> graphs;
 };
 
 class Server {
@@ -81,7 +89,8 @@ class Server {
   private:
     void run(size_t e);
     bool next_batch(std::vector<Segment> &batch);
-    void execute(Executor &ex, std::vector<Segment> &batch);
+    // runs the batch; returns true when it replayed a captured graph
+    bool execute(Executor &ex, std::vector<Segment> &batch);
 
     std::shared_ptr<GpuIndex> index_;
     nndex_server_options opt_;
@@ -97,7 +106,8 @@ class Server {
     size_t slot_bytes_ = 0;
     // statistics since the last reset
     std::vector<double> lat_ms_;
-    uint64_t s_requests_ = 0, s_queries_ = 0, s_batches_ = 0, s_batch_rows_ = 0;
+    uint64_t s_requests_ = 0, s_queries_ = 0, s_batches_ = 0, s_batch_rows_ = 0,
+             s_graph_batches_ = 0;
     Clock::time_point s_first_, s_last_;
     bool s_any_ = false;
 };
@@ -115,7 +125,7 @@ Server::Server(std::shared_ptr<GpuIndex> index, const nndex_server_options &o)
     const size_t np = opt_.ivf_nprobe_max;
     NNDEX_HIP_CHECK(hipSetDevice(index_->device()));
     // footprint of one executor: workspace + device query/result buffers
-    slot_bytes_ = index_->workspace_bytes(B, K, np) + B * pdims_ * 4 + B * K * 12 + B * 8;
+    slot_bytes_ = index_->workspace_bytes(B, K, np) + B * pdims_ * 4 + B * K * 12 + B * 9;
     size_t slots = opt_.hbm_budget_bytes / slot_bytes_;
     if (slots > opt_.slots) slots = opt_.slots;
     if (slots == 0)
@@ -133,6 +143,8 @@ Server::Server(std::shared_ptr<GpuIndex> index, const nndex_server_options &o)
         ex->didx.reserve(B * K * 8);
         ex->dsim.reserve(B * K * 4);
         ex->dlen.reserve(B * 8);
+        ex->dflags.reserve(B);
+        ex->hflags.reserve(B);
         ex->hq.reserve(B * pdims_ * 4);
         ex->hidx.reserve(B * K * 8);
         ex->hsim.reserve(B * K * 4);
@@ -226,7 +238,7 @@ bool Server::next_batch(std::vector<Segment> &batch) {
     return true;
 }
 
-void Server::execute(Executor &ex, std::vector<Segment> &batch) {
+bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
     uint32_t nq = 0, k = 0;
     for (auto &sg : batch) {
         nq += sg.n;
@@ -241,25 +253,60 @@ void Server::execute(Executor &ex, std::vector<Segment> &batch) {
                     (size_t)sg.n * pdims_ * sizeof(float));
         off += sg.n;
     }
-    NNDEX_HIP_CHECK(hipMemcpyAsync(ex.dq.ptr, hq, (size_t)nq * pdims_ * 4, hipMemcpyHostToDevice,
-                                   ex.stream));
     const float *dq = static_cast<const float *>(ex.dq.ptr);
     uint64_t *didx = static_cast<uint64_t *>(ex.didx.ptr);
     float *dsim = static_cast<float *>(ex.dsim.ptr);
-    if (mode == 0)
-        index_->search_exact_any(*ex.ws, dq, nq, k, didx, dsim, ex.stream);
-    else
-        index_->search_ivf_device(*ex.ws, dq, nq, k, nprobe, didx, dsim,
-                                  static_cast<uint64_t *>(ex.dlen.ptr), ex.stream);
-    NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hidx.ptr, didx, (size_t)nq * k * 8, hipMemcpyDeviceToHost,
-                                   ex.stream));
-    NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hsim.ptr, dsim, (size_t)nq * k * 4, hipMemcpyDeviceToHost,
-                                   ex.stream));
+    uint32_t kstride = k;  // result row stride of this batch
+    bool graphed = false;
+    Precision prec;
+    if (mode == 0 && opt_.use_graphs && nq > 8 && index_->exact_batch_precision(prec)) {
+        // replay a captured exact batch: rows padded to a power of two (with
+        // copies of the last row), k to a power of two (results are prefixes)
+        uint32_t qb = 16, kb = 1;
+        while (qb < nq) qb <<= 1;
+        if (qb > opt_.max_batch_queries) qb = opt_.max_batch_queries;
+        while (kb < k) kb <<= 1;
+        if (kb > opt_.max_k) kb = opt_.max_k;
+        for (uint32_t r = nq; r < qb; ++r)
+            std::memcpy(hq + (size_t)r * pdims_, hq + (size_t)(nq - 1) * pdims_, pdims_ * 4);
+        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.dq.ptr, hq, (size_t)qb * pdims_ * 4,
+                                       hipMemcpyHostToDevice, ex.stream));
+        auto &g = ex.graphs[{qb, kb}];
+        uint8_t *dflags = static_cast<uint8_t *>(ex.dflags.ptr);
+        // (re)capture when new, or when a workspace buffer moved since
+        if (!g || g->generation != ex.ws->generation)
+            g = index_->capture_batch(*ex.ws, dq, qb, kb, prec, 0, didx, dsim, dflags, ex.stream);
+        g->launch(ex.stream);
+        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hflags.ptr, dflags, qb, hipMemcpyDeviceToHost,
+                                       ex.stream));
+        NNDEX_HIP_CHECK(hipStreamSynchronize(ex.stream));
+        // rows the certificate left open: the exact fused scan, as in the
+        // exact batch search
+        const uint8_t *hf = static_cast<const uint8_t *>(ex.hflags.ptr);
+        for (uint32_t q = 0; q < nq; ++q)
+            if (hf[q])
+                index_->search_device(*ex.ws, dq + (size_t)q * pdims_, 1, kb,
+                                      didx + (size_t)q * kb, dsim + (size_t)q * kb, ex.stream);
+        kstride = kb;
+        graphed = true;
+    } else {
+        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.dq.ptr, hq, (size_t)nq * pdims_ * 4,
+                                       hipMemcpyHostToDevice, ex.stream));
+        if (mode == 0)
+            index_->search_exact_any(*ex.ws, dq, nq, k, didx, dsim, ex.stream);
+        else
+            index_->search_ivf_device(*ex.ws, dq, nq, k, nprobe, didx, dsim,
+                                      static_cast<uint64_t *>(ex.dlen.ptr), ex.stream);
+    }
+    NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hidx.ptr, didx, (size_t)nq * kstride * 8,
+                                   hipMemcpyDeviceToHost, ex.stream));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hsim.ptr, dsim, (size_t)nq * kstride * 4,
+                                   hipMemcpyDeviceToHost, ex.stream));
     if (mode == 1)
         NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hlen.ptr, ex.dlen.ptr, (size_t)nq * 8,
                                        hipMemcpyDeviceToHost, ex.stream));
     NNDEX_HIP_CHECK(hipStreamSynchronize(ex.stream));
-    // scatter: each request keeps the first k of its rows' k_batch results
+    // scatter: each request keeps the first k of its rows' results
     const uint64_t *hi = static_cast<const uint64_t *>(ex.hidx.ptr);
     const float *hs = static_cast<const float *>(ex.hsim.ptr);
     const uint64_t *hl = static_cast<const uint64_t *>(ex.hlen.ptr);
@@ -267,14 +314,15 @@ void Server::execute(Executor &ex, std::vector<Segment> &batch) {
     for (auto &sg : batch) {
         Request &r = *sg.req;
         for (uint32_t i = 0; i < sg.n; ++i) {
-            const size_t src = (off + i) * k, dst = (size_t)(sg.q0 + i) * r.k;
+            const size_t src = (off + i) * kstride, dst = (size_t)(sg.q0 + i) * r.k;
             std::memcpy(&r.idx[dst], hi + src, r.k * 8);
             std::memcpy(&r.sim[dst], hs + src, r.k * 4);
-            const uint64_t found = mode == 1 ? hl[off + i] : k;
+            const uint64_t found = mode == 1 ? hl[off + i] : kstride;
             r.lens[sg.q0 + i] = found < r.k ? found : r.k;
         }
         off += sg.n;
     }
+    return graphed;
 }
 
 void Server::run(size_t e) {
@@ -295,8 +343,9 @@ void Server::run(size_t e) {
         }
         nndex_status st = NNDEX_OK;
         std::string err;
+        bool graphed = false;
         try {
-            execute(ex, batch);
+            graphed = execute(ex, batch);
         } catch (const Error &x) {
             st = x.code;
             err = x.what();
@@ -333,6 +382,7 @@ void Server::run(size_t e) {
             }
             ++s_batches_;
             s_batch_rows_ += rows;
+            if (graphed) ++s_graph_batches_;
         }
         cv_done_.notify_all();
         if (st != NNDEX_OK) cv_work_.notify_all();
@@ -376,6 +426,7 @@ void Server::stats(nndex_server_stats &out) {
     out.requests = s_requests_;
     out.queries = s_queries_;
     out.batches = s_batches_;
+    out.graph_batches = s_graph_batches_;
     out.mean_batch_queries = s_batches_ ? (double)s_batch_rows_ / s_batches_ : 0.0;
     out.p50_ms = percentile(lat_ms_, 50);
     out.p99_ms = percentile(lat_ms_, 99);
@@ -390,7 +441,7 @@ void Server::stats(nndex_server_stats &out) {
 void Server::reset_stats() {
     std::lock_guard<std::mutex> l(mu_);
     lat_ms_.clear();
-    s_requests_ = s_queries_ = s_batches_ = s_batch_rows_ = 0;
+    s_requests_ = s_queries_ = s_batches_ = s_batch_rows_ = s_graph_batches_ = 0;
     s_any_ = false;
 }
 

@@ -26,19 +26,20 @@ class Server:
     """Serve searches on a backend handle (``nndex_gpu_index*``).
 
     Queries are normalised, zero-padded rows (``padded_dims`` columns), as at
-    the backend boundary.  ``submit`` returns a ticket at once; ``wait``
+    the backend boundary.  ``graphs=True`` replays captured HIP graphs for exact
+    batches above 8 rows (rows and k rounded up to powers of two).  ``submit`` returns a ticket at once; ``wait``
     returns ``(indices[uint64, rows x k], similarities[float32, rows x k],
     counts[uint64, rows], latency_ms)``.
     """
 
     def __init__(self, backend_handle, padded_dims: int, rows: int, slots: int = 4,
                  max_batch_queries: int = 2048, max_k: int = 100, ivf_nprobe_max: int = 0,
-                 hbm_budget_bytes: int = 0):
+                 hbm_budget_bytes: int = 0, graphs: bool = False):
         self._L = _lib.lib()
         self.pdims = int(padded_dims)
         self.rows = int(rows)
         opt = _lib.ServerOptions(slots, max_batch_queries, max_k, ivf_nprobe_max,
-                                 hbm_budget_bytes)
+                                 hbm_budget_bytes, 1 if graphs else 0)
         self._h = ctypes.c_void_p()
         _lib.check(self._L.nndex_server_new(backend_handle, ctypes.byref(opt),
                                             ctypes.byref(self._h)))

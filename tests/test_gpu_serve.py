@@ -1,7 +1,7 @@
 """Serving loop: continuous batching gives every request the same result it
 gets alone (exact requests bit for bit equal to the exact search, IVF requests
-to the IVF search), across mixed kinds, k and sizes, split requests and
-concurrent submitters."""
+to the IVF search), across mixed kinds, k and sizes, split requests, concurrent
+submitters, and exact batches replayed from captured HIP graphs."""
 
 import threading
 import time
@@ -84,13 +84,15 @@ def test_serve_alone_equals_direct(setup):
 def test_serve_mixed_equals_alone(setup):
     b, rows, dims, reqs = setup
     _case("serve_mixed", requests=len(reqs))
+    # alone: eager launches; mixed: exact batches replay captured HIP graphs
     alone = Server(b.h, dims, rows, slots=1, max_batch_queries=2048, max_k=100,
-                   ivf_nprobe_max=16)
+                   ivf_nprobe_max=16, graphs=False)
     ref = [alone.wait(alone.submit(q, k, m, p)) for q, k, m, p in reqs]
     alone.close()
     # three executors, batches capped at 256 rows (large requests split), all
     # requests queued at once: they merge, split and overlap
-    srv = Server(b.h, dims, rows, slots=3, max_batch_queries=256, max_k=100, ivf_nprobe_max=16)
+    srv = Server(b.h, dims, rows, slots=3, max_batch_queries=256, max_k=100, ivf_nprobe_max=16,
+                 graphs=True)
     tickets = [srv.submit(q, k, m, p) for q, k, m, p in reqs]
     got = [srv.wait(t) for t in tickets]
     for r, g in zip(ref, got):
@@ -100,6 +102,7 @@ def test_serve_mixed_equals_alone(setup):
     assert st["requests"] == len(reqs)
     assert st["queries"] == sum(q.shape[0] for q, _, _, _ in reqs)
     assert st["slots"] == 3 and st["batches"] >= 1
+    assert st["graph_batches"] >= 1
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     srv.close()
 
