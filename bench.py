@@ -109,6 +109,7 @@ def ivf_bench(args, k: int = 10) -> dict:
     h = ctypes.c_void_p()
     check(L.nndex_gpu_index_new(base.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rows, dims,
                                 1, 0, -1, ctypes.byref(h)))
+    small = np.ascontiguousarray(base[:args.ivf_cpu_rows]) if not args.no_cpu else None
     del base
     check(L.nndex_gpu_index_set_ivf_params(h, nlist, 0, args.ivf_iters))
     info = _lib.IvfInfo()
@@ -166,7 +167,74 @@ def ivf_bench(args, k: int = 10) -> dict:
     L.nndex_stream_destroy(stream)
     L.nndex_workspace_free(ws)
     L.nndex_gpu_index_free(h)
+    if small is not None:
+        res["cpu_baseline"] = ivf_cpu_baseline(small, queries[:256], k)
     return res
+
+
+def ivf_cpu_baseline(m: np.ndarray, queries: np.ndarray, k: int) -> dict:
+    """The reference IVF (oracle restatement of src/approx.rs with the GPU
+    backend's search, src/gpu.rs:448-462) against the GPU IVF on the same rows
+    with the reference's own settings (choose_nlist, 12 iterations,
+    choose_nprobe): build and search time on each, and whether centroids,
+    assignments and every result are identical."""
+    import oracle
+
+    L = _lib.lib()
+    rows, dims = m.shape
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ref = oracle.Ivf(m, threads=threads)
+    cpu_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref_res = [ref.search_gpu(q, k) for q in queries]
+    cpu_search = time.perf_counter() - t0
+    h = ctypes.c_void_p()
+    check(L.nndex_gpu_index_new(m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rows, dims, 1,
+                                0, -1, ctypes.byref(h)))
+    info = _lib.IvfInfo()
+    t0 = time.perf_counter()
+    check(L.nndex_gpu_index_build_ivf(h, ctypes.byref(info)))
+    gpu_build = time.perf_counter() - t0
+    nl = int(info.nlist)
+    cent = np.zeros((nl, dims), np.float32)
+    assign = np.zeros(rows, np.uint64)
+    check(L.nndex_gpu_index_ivf_export(h, cent.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       None, None,
+                                       assign.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    same_build = (np.array_equal(cent.view(np.uint32), ref.centroids().view(np.uint32)) and
+                  np.array_equal(assign, ref.assignments()))
+    nq = queries.shape[0]
+    idx = np.zeros((nq, k), np.uint64)
+    sim = np.zeros((nq, k), np.float32)
+    lens = np.zeros(nq, np.uintp)
+    qc = np.ascontiguousarray(queries, np.float32)
+
+    def search():
+        check(L.nndex_gpu_index_search_batch(
+            h, qc.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nq, k, 1,
+            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            sim.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t))))
+    search()
+    t0 = time.perf_counter()
+    search()
+    gpu_search = time.perf_counter() - t0
+    same_search = all(int(lens[i]) == len(ref_res[i][0]) and
+                      np.array_equal(idx[i, :lens[i]], ref_res[i][0]) and
+                      np.array_equal(sim[i, :lens[i]].view(np.uint32),
+                                     ref_res[i][1].view(np.uint32))
+                      for i in range(nq))
+    L.nndex_gpu_index_free(h)
+    out = {"kind": "port", "cores": threads,
+           "sample": f"{rows}x{dims} (the first rows of the base), reference settings: nlist "
+                     f"{nl}, 12 iterations, nprobe choose_nprobe(k={k}); {nq} queries; CPU "
+                     "search single-threaded per query as the reference's host path",
+           "cpu_build_s": round(cpu_build, 3), "gpu_build_s": round(gpu_build, 3),
+           "cpu_qps": round(nq / cpu_search, 1), "gpu_host_api_qps": round(nq / gpu_search, 1),
+           "identical_build": bool(same_build), "identical_results": bool(same_search)}
+    log(f"ivf cpu baseline: {out}")
+    return out
 
 
 def serve_bench(args) -> dict:
@@ -345,6 +413,8 @@ def main():
     ap.add_argument("--ivf-nlist", type=int, default=4096)
     ap.add_argument("--ivf-iters", type=int, default=20)
     ap.add_argument("--ivf-queries", type=int, default=1024)
+    ap.add_argument("--ivf-cpu-rows", type=int, default=200_000,
+                    help="rows of the reduced IVF case timed on the CPU oracle and the GPU")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--precision-only", action="store_true",
                     help="run only the precision form and print its record")
