@@ -52,6 +52,9 @@ struct F32E {
         return (c[0][v] + c[1][v]) + (c[2][v] + c[3][v]);
     }
     static constexpr bool SCALED = false, PAIR = false;
+    // the skewed epilogue only pays where MFMA steps are short (16-bit and
+    // 8-bit forms); four f32 MFMAs per step already hide the epilogue
+    static constexpr bool SKEW = false;
     static __device__ inline void mma2(uint4, uint4, uint4, uint4, v4f (&)[NACC]) {}
 };
 struct BF16E {
@@ -62,7 +65,7 @@ struct BF16E {
                                                        __builtin_bit_cast(v8bf, b), c[0], 0, 0, 0);
     }
     static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
-    static constexpr bool SCALED = false, PAIR = false;
+    static constexpr bool SCALED = false, PAIR = false, SKEW = true;
     static __device__ inline void mma2(uint4, uint4, uint4, uint4, v4f (&)[NACC]) {}
 };
 struct FP8E {
@@ -72,7 +75,7 @@ struct FP8E {
     // on both sides; the score is rescaled by the row and query scales in the
     // epilogue
     static constexpr int NACC = 1;
-    static constexpr bool SCALED = true, PAIR = true;
+    static constexpr bool SCALED = true, PAIR = true, SKEW = true;
     static __device__ inline void mma(uint4, uint4, v4f (&)[NACC]) {}
     static __device__ inline void mma2(uint4 a0, uint4 a1, uint4 b0, uint4 b1, v4f (&c)[NACC]) {
         const v8i A = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
@@ -346,73 +349,167 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // buffer.  Mode 0 appends candidates, mode 1 writes scores; the AM variant
 // (mode 2) keeps each row's two best queries and writes the row argmax.
 // ---------------------------------------------------------------------------
+// Entries of a wave's LDS candidate queue (12 bytes each).
+constexpr uint32_t kCandQueue = 128;
+
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM>
-__global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
+__global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
     constexpr uint32_t QC = 16 * NT;     // queries per chunk
-    constexpr uint32_t kStride = QC + 1; // LDS slots per unit row (bank spread)
+    // LDS slots per unit row: unpadded, so the 16 lanes of a group read one
+    // aligned 256-byte row; query n of unit row u sits at slot
+    // (n & ~15) | ((n ^ u) & 15), which keeps those reads in the row and
+    // spreads the transposed commit writes over the banks
+    constexpr uint32_t kStride = QC;
     constexpr uint32_t NTH = NW * 64;
     constexpr int STG = (int)((QC * 4 * MAXS + NTH - 1) / NTH);  // staged units per thread
-    extern __shared__ uint4 qlds[];      // 2 buffers x [4*steps units][kStride]
+    constexpr uint32_t TU = QC / 2;      // threshold units (two 64-bit keys each) per buffer
+    extern __shared__ uint4 qlds[];      // 2 buffers x ([4*steps units][kStride] + TU)
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t wid = tid >> 6;
     const uint32_t g = lane >> 4, l16 = lane & 15;
     const uint32_t dch = a.dch;
     constexpr uint32_t steps = MAXS;  // launcher guarantees 4 * MAXS >= dch
-    const uint32_t buf_units = steps * 4 * kStride;
+    constexpr uint32_t q_units = steps * 4 * kStride;
+    constexpr uint32_t buf_units = q_units + TU;
     const uint32_t block_rows = NW * RW;
     const uint32_t nblk = (a.rows_padded + block_rows - 1) / block_rows;
     const uint32_t nchunks = (a.nq + QC - 1) / QC;
     const uint32_t chunk_units = QC * dch;
+    if (a.nq == 0 || dch == 0) return;
+    auto thl = [&](uint32_t buf) {
+        return reinterpret_cast<unsigned long long *>(qlds + buf * buf_units + q_units);
+    };
+    // Mode 0 candidate queue of the wave in LDS (after both buffers): keys
+    // and query ids of the pairs that passed, flushed to the per-query lists
+    // when full and at the end of each row tile, so the epilogue never waits
+    // on a returning atomic.
+    uint64_t *cq_key = reinterpret_cast<uint64_t *>(qlds + 2 * buf_units) + wid * kCandQueue;
+    uint32_t *cq_qi = reinterpret_cast<uint32_t *>(qlds + 2 * buf_units + NW * kCandQueue / 2) +
+                      wid * kCandQueue;
+    uint32_t cq_n = 0;  // wave-uniform
+    auto flush = [&]() {
+        wave_lds_sync();
+        for (uint32_t i = lane; i < cq_n; i += 64) {
+            const uint32_t qi = cq_qi[i];
+            const uint64_t key = cq_key[i];
+            const uint32_t pos = atomicAdd(a.cand_count + qi, 1u);
+            if (pos < a.capq) a.cand[(size_t)qi * a.capq + pos] = key;
+        }
+        wave_lds_sync();
+        cq_n = 0;
+    };
 
     // zero the padding units of both buffers once (units >= dch)
     for (uint32_t i = tid; i < 2 * buf_units; i += NTH) {
-        uint32_t unit = (i % buf_units) / kStride;
-        if (unit >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
+        const uint32_t r = i % buf_units;
+        if (r < q_units && r / kStride >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
     }
 
+    // Staging of the next chunk (queries and their thresholds).  The loads
+    // are unconditional (clamped addresses; out-of-range values are dropped
+    // at the commit), so the count of loads in flight is the same on every
+    // path and the first MFMA of a chunk does not wait for them.
+    const unsigned long long *thsrc =
+        a.mode == 0 ? a.thresh : reinterpret_cast<const unsigned long long *>(a.q);
     uint4 stg[STG];
+    unsigned long long stg_th;
     auto fetch = [&](uint32_t ch) {
 #pragma unroll
         for (int j = 0; j < STG; ++j) {
             const uint32_t i = tid + j * NTH;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (i < chunk_units) {
-                const uint32_t n = i / dch, c = i % dch, qi = ch * QC + n;
-                if (qi < a.nq) v = a.q[(size_t)qi * a.q_stride + c];
-            }
-            stg[j] = v;
+            const uint32_t ic = i < chunk_units ? i : chunk_units - 1;
+            const uint32_t n = ic / dch, c = ic - n * dch;
+            uint32_t qi = ch * QC + n;
+            qi = qi < a.nq ? qi : a.nq - 1;
+            stg[j] = a.q[(size_t)qi * a.q_stride + c];
         }
+        uint32_t qt = ch * QC + tid % QC;
+        qt = qt < a.nq ? qt : a.nq - 1;
+        stg_th = thsrc[qt];
     };
-    auto commit = [&](uint32_t buf) {
+    auto commit = [&](uint32_t buf, uint32_t ch) {
         uint4 *dst = qlds + buf * buf_units;
 #pragma unroll
         for (int j = 0; j < STG; ++j) {
             const uint32_t i = tid + j * NTH;
-            if (i < chunk_units) dst[(i % dch) * kStride + i / dch] = stg[j];
+            if (i < chunk_units) {
+                const uint32_t n = i / dch, c = i - n * dch;
+                dst[c * kStride + ((n & ~15u) | ((n ^ c) & 15u))] =
+                    ch * QC + n < a.nq ? stg[j] : make_uint4(0, 0, 0, 0);
+            }
         }
+        if (tid < QC)
+            thl(buf)[tid] = (a.mode == 0 && ch * QC + tid < a.nq) ? stg_th : ~0ull;
     };
-    auto fetch_thresh = [&](uint32_t ch, unsigned long long (&th)[NT]) {
+
+
+    // Mode 0 epilogue of one chunk: append the (row, query) pairs whose keys
+    // reach the query's threshold to the wave's queue.  Scores are finite
+    // (finite inputs).  A key at or above the threshold has a score at or
+    // above the threshold's floor, so each 16-query tile first tests the
+    // largest score of every lane and is skipped when none is that high.
+    auto epi0 = [&](const v4f (&ac)[MT][NT][E::NACC], const unsigned long long (&thk)[NT],
+                    uint32_t ch, uint32_t bt, const float (&rs)[MT][4]) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const uint32_t qi = ch * QC + t * 16 + l16;
-            th[t] = (a.mode == 0 && qi < a.nq) ? a.thresh[qi] : ~0ull;
+            float qs = 1.f;
+            if constexpr (E::SCALED) qs = qi < a.nq ? a.q_scale[qi] : 0.f;
+            const float thf = key_floor_sim(thk[t]);
+            float sc[MT][4];
+            float mx = -__builtin_inff();
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    sc[m][v] = E::total(ac[m][t], v);
+                    if constexpr (E::SCALED) sc[m][v] = sc[m][v] * (qs * rs[m][v]);
+                    mx = __builtin_fmaxf(mx, sc[m][v]);
+                }
+            if (__ballot(!(mx < thf)) == 0) continue;
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
+                    // above the floor passes; at the floor the row part of
+                    // the key decides
+                    bool pass = sc[m][v] > thf;
+                    if (sc[m][v] == thf) pass = make_key(sc[m][v], row) >= thk[t];
+                    pass = pass && qi < a.nq && row < a.rows;
+                    const uint64_t hm = __ballot(pass);
+                    if (hm == 0) continue;
+                    const uint32_t n = __popcll(hm);
+                    if (cq_n + n > kCandQueue) flush();
+                    const uint32_t pos = cq_n + __popcll(hm & __lanemask_lt());
+                    if (pass) {
+                        cq_key[pos] = make_key(sc[m][v], row);
+                        cq_qi[pos] = qi;
+                    }
+                    cq_n += n;
+                }
         }
     };
-
+    // Half of the waves (one of each SIMD's pair) run the mode 0 epilogue one
+    // chunk late, so one wave's epilogue overlaps the other's MFMAs.
+    const bool late = E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
+    v4f accp[MT][NT][E::NACC];
+    unsigned long long thkp[NT];
     for (uint32_t bt = blockIdx.x; bt < nblk; bt += gridDim.x) {
-        // ---- stationary X: MT x MAXS units per lane ----
+        // ---- stationary X: MT x MAXS units per lane (rows past the padded
+        // count read the last row; their results are never written) ----
         uint4 xr[MT][MAXS];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
-            const uint32_t row = bt * block_rows + wid * RW + m * 16 + l16;
-            const bool in = row < a.rows_padded;
+            uint32_t row = bt * block_rows + wid * RW + m * 16 + l16;
+            row = row < a.rows_padded ? row : a.rows_padded - 1;
             const uint4 *base = a.x + (size_t)(row / 64) * dch * 64 + (row % 64);
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
                 const uint32_t c = 4 * s + g;
-                xr[m][s] = (in && c < dch) ? base[(size_t)c * 64] : make_uint4(0, 0, 0, 0);
+                xr[m][s] = c < dch ? base[(size_t)c * 64] : make_uint4(0, 0, 0, 0);
             }
         }
         uint64_t best[MT][4], second[MT][4];  // row argmax variant only
@@ -428,22 +525,21 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 rs[m][v] = 1.f;
-                if constexpr (E::SCALED)
-                    rs[m][v] = a.row_scale[bt * block_rows + wid * RW + m * 16 + 4 * g + v];
+                if constexpr (E::SCALED) {
+                    const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
+                    rs[m][v] = a.row_scale[row < a.rows_padded ? row : a.rows_padded - 1];
+                }
             }
         __syncthreads();  // previous block tile done with both buffers
         fetch(0);
-        commit(0);
-        unsigned long long th_cur[NT], th_next[NT];
-        fetch_thresh(0, th_cur);
+        commit(0, 0);
         __syncthreads();
         for (uint32_t ch = 0; ch < nchunks; ++ch) {
             const bool more = ch + 1 < nchunks;
-            if (more) {
-                fetch(ch + 1);
-                fetch_thresh(ch + 1, th_next);
-            }
+            fetch(more ? ch + 1 : ch);
+            const unsigned long long *thc = thl(ch & 1);
             const uint4 *src = qlds + (ch & 1) * buf_units;
+            if (late && ch > 0) epi0(accp, thkp, ch - 1, bt, rs);
             v4f acc[MT][NT][E::NACC];
 #pragma unroll
             for (int m = 0; m < MT; ++m)
@@ -451,42 +547,68 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
                 for (int t = 0; t < NT; ++t)
 #pragma unroll
                     for (int j = 0; j < E::NACC; ++j) acc[m][t][j] = v4f{0.f, 0.f, 0.f, 0.f};
-            // one-step software pipeline over LDS reads; the scheduling
-            // barriers keep the compiler from hoisting every step's reads.
-            // An MMA covers SW steps (SW = 2 for the K=128 FP8 form).
+            // LDS reads run PD step groups ahead of the MFMAs; the scheduling
+            // barriers keep the compiler from hoisting every group's reads.
+            // An MMA covers SW steps (SW = 2 for the K=128 FP8 form).  Unit
+            // row u = 4 S + g of step S sits at lane slot (l16 ^ u) & 15 =
+            // l16 ^ (4 (S & 3) + g): four base pointers, constant offsets.
             constexpr int SW = E::PAIR ? 2 : 1;
-            uint4 qc[SW][NT], qn[SW][NT];
+            constexpr int NG = MAXS / SW;
+            constexpr int PD = NG > 2 ? 2 : 1;
+            const uint4 *sb[4];
 #pragma unroll
-            for (int i = 0; i < SW; ++i)
+            for (int k = 0; k < 4; ++k)
+                sb[k] = src + g * kStride + ((l16 ^ (4 * k + g)) & 15);
+            uint4 qb[PD + 1][SW][NT];
+            auto load_group = [&](int grp, uint4 (&dst)[SW][NT]) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) qc[i][t] = src[(4 * i + g) * kStride + t * 16 + l16];
+                for (int i = 0; i < SW; ++i)
 #pragma unroll
-            for (int s = 0; s < MAXS; s += SW) {
-                if (s + SW < MAXS) {
+                    for (int t = 0; t < NT; ++t) {
+                        const int st = grp * SW + i;
+                        dst[i][t] = sb[st & 3][4 * st * kStride + t * 16];
+                    }
+            };
 #pragma unroll
-                    for (int i = 0; i < SW; ++i)
+            for (int p = 0; p < PD; ++p) load_group(p, qb[p]);
 #pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            qn[i][t] = src[(4 * (s + SW + i) + g) * kStride + t * 16 + l16];
-                }
+            for (int grp = 0; grp < NG; ++grp) {
+                if (grp + PD < NG) load_group(grp + PD, qb[(grp + PD) % (PD + 1)]);
+                const uint4 (&qc)[SW][NT] = qb[grp % (PD + 1)];
+                const int st = grp * SW;
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
 #pragma unroll
                     for (int m = 0; m < MT; ++m) {
                         if constexpr (E::PAIR)
-                            E::mma2(xr[m][s], xr[m][s + 1], qc[0][t], qc[1][t], acc[m][t]);
+                            E::mma2(xr[m][st], xr[m][st + 1], qc[0][t], qc[1][t], acc[m][t]);
                         else
-                            E::mma(xr[m][s], qc[0][t], acc[m][t]);
+                            E::mma(xr[m][st], qc[0][t], acc[m][t]);
                     }
-#pragma unroll
-                for (int i = 0; i < SW; ++i)
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) qc[i][t] = qn[i][t];
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // ---- epilogue: lane holds rows 4*g+v of each tile, query 16t+l16 ----
+            if (!AM && a.mode == 0) {
+                unsigned long long thk[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) thk[t] = thc[t * 16 + l16];
+                if (late) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        thkp[t] = thk[t];
+#pragma unroll
+                        for (int m = 0; m < MT; ++m)
+#pragma unroll
+                            for (int j = 0; j < E::NACC; ++j) accp[m][t][j] = acc[m][t][j];
+                    }
+                } else {
+                    epi0(acc, thk, ch, bt, rs);
+                }
+            }
+            // ---- mode 1 and row argmax epilogue: lane holds rows 4*g+v of
+            // each tile, query 16t+l16 ----
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
+                if (!AM && a.mode == 0) break;
                 const uint32_t qi = ch * QC + t * 16 + l16;
                 float qs = 1.f;
                 if constexpr (E::SCALED) qs = qi < a.nq ? a.q_scale[qi] : 0.f;
@@ -505,25 +627,18 @@ __global__ __launch_bounds__(NW * 64) void mfma_scan_kernel(MfmaScanArgs a) {
                             } else if (key > second[m][v]) {
                                 second[m][v] = key;
                             }
-                        } else if (a.mode == 0) {
-                            const uint64_t key = make_key(sc, row);
-                            const bool pass = qi < a.nq && row < a.rows && key >= th_cur[t];
-                            if (__ballot(pass) != 0 && pass) {
-                                uint32_t pos = atomicAdd(a.cand_count + qi, 1u);
-                                if (pos < a.capq) a.cand[(size_t)qi * a.capq + pos] = key;
-                            }
                         } else if (qi < a.nq && row < a.rows) {
                             a.out[(size_t)qi * a.ldo + row] = sc;
                         }
                     }
                 }
             }
-            if (more) {
-                commit((ch + 1) & 1);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) th_cur[t] = th_next[t];
-            }
+            if (more) commit((ch + 1) & 1, ch + 1);
             __syncthreads();
+        }
+        if constexpr (!AM) {
+            if (late) epi0(accp, thkp, nchunks - 1, bt, rs);
+            if (cq_n) flush();
         }
         if constexpr (AM) {
             // top two over the 16 lanes that hold the same rows (one query
@@ -667,6 +782,11 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
             // t_low).  With no left-out rows the answer is exact outright.
             const unsigned long long th = a.thresh[q];
             bool ok = !overflow && enough;
+            // a tie across the k-th score: the reference's heap picks among
+            // the tied rows by arrival order, which the exact scan reproduces
+            if (ok && a.k > 1 && nres > a.k &&
+                (uint32_t)(exact[a.k] >> 32) == (uint32_t)(exact[a.k - 1] >> 32))
+                ok = false;
             const bool left_out = m > a.kp || th != 0;
             if (ok && left_out) {
                 const double bound = m > a.kp ? (double)key_sim(slots[0][a.kp])
@@ -693,7 +813,7 @@ __global__ __launch_bounds__(256) void rerank_full_kernel(RerankArgs a) {
     const uint64_t *c = a.cand + (size_t)q * a.capq;
     const float4 *qb = a.q + (size_t)q * a.d4;
     WaveTopK tk;
-    tk.init(slots[wid], CW, a.k, nullptr);
+    tk.init(slots[wid], CW, a.k + 1, nullptr);  // one past k: ties across the k-th
     for (uint32_t i0 = wid * 64; i0 < total; i0 += 256) {
         const uint32_t i = i0 + lane;
         uint64_t key = 0;
@@ -727,6 +847,9 @@ __global__ __launch_bounds__(256) void rerank_full_kernel(RerankArgs a) {
     const double t_low = th ? (double)from_ordered((uint32_t)(th >> 32)) : -1e300;
     const bool ok = th == 0 || t_low + (double)a.delta[q] < (double)key_sim(slots[0][a.k - 1]);
     if (!ok) return;
+    if (a.k > 1 && tk.count > a.k &&
+        (uint32_t)(slots[0][a.k] >> 32) == (uint32_t)(slots[0][a.k - 1] >> 32))
+        return;  // tie across the k-th score: left to the exact scan
     for (uint32_t i = lane; i < a.k; i += 64) {
         a.out_idx[(size_t)q * a.k + i] = key_row(slots[0][i]);
         a.out_sim[(size_t)q * a.k + i] = key_sim(slots[0][i]);
@@ -855,7 +978,9 @@ void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, f
 
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
 static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
-    const size_t lds = 2ull * MAXS * 4 * (16 * NT + 1) * sizeof(uint4);
+    // two query/threshold buffers, then (mode 0) the waves' candidate queues
+    const size_t lds = (2ull * (MAXS * 4 * 16 * NT + 8 * NT) +
+                        (AM ? 0 : NW * kCandQueue * 12 / 16)) * sizeof(uint4);
     auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM>;
     // once per instantiation (also keeps the call out of graph captures)
     static bool attr_set = false;
@@ -868,55 +993,69 @@ static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-// Step counts with an instantiation; a row of `dch` 16-byte units runs the
-// smallest one with 4 * steps >= dch (extra units are zero).
-static uint32_t mfma_steps(uint32_t dch) {
-    for (uint32_t st : {2u, 4u, 6u, 8u, 12u, 16u, 24u, 32u})
-        if (4 * st >= dch) return st;
-    return 0;
+// Tile shapes per step count (a row of `dch` 16-byte units runs the smallest
+// step count with 4 * steps >= dch; extra units are zero): the 16-query MFMA
+// tiles per chunk (nt) and rows per wave (rw) of each element type, for the
+// candidate and score scans and, separately, for the row argmax (whose
+// epilogue keeps two keys per row).  Chosen by timing at 1M x 128-1024 with
+// 2048 queries; every shape fits 256 VGPRs.
+struct MfmaTiles {
+    uint32_t steps;
+    int f32_nt, bf_rw, bf_nt, f8_rw, f8_nt;  // mode 0 / 1
+    int am_f32_nt, am_bf_rw, am_bf_nt;       // mode 2
+};
+static constexpr MfmaTiles kMfmaTiles[] = {
+    {2, 4, 32, 4, 32, 4, 4, 32, 4},   {4, 4, 32, 4, 32, 2, 4, 32, 4},
+    {6, 2, 32, 2, 32, 2, 2, 32, 2},   {8, 2, 32, 2, 16, 2, 2, 32, 2},
+    {12, 2, 32, 2, 32, 2, 2, 16, 4},  {16, 2, 32, 1, 32, 1, 2, 16, 2},
+    {24, 2, 16, 2, 16, 1, 1, 16, 1},  {32, 1, 16, 1, 16, 1, 1, 16, 1},
+};
+constexpr int kMfmaTileCount = sizeof(kMfmaTiles) / sizeof(kMfmaTiles[0]);
+
+static int mfma_tile_index(uint32_t dch) {
+    for (int i = 0; i < kMfmaTileCount; ++i)
+        if (4 * kMfmaTiles[i].steps >= dch) return i;
+    return -1;
 }
 
-uint32_t mfma_block_rows(int elem, uint32_t dch) {
-    const uint32_t st = mfma_steps(dch);
-    if (elem == kElemFP8) return st <= 6 ? 8 * 32 : 8 * 16;
-    return elem == kElemBF16 && st <= 8 ? 8 * 32 : 8 * 16;
+uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax) {
+    const int i = mfma_tile_index(dch);
+    if (i < 0) return 8 * 16;
+    const MfmaTiles &t = kMfmaTiles[i];
+    if (elem == kElemFP8) return 8 * t.f8_rw;
+    if (elem == kElemBF16) return 8 * (argmax ? t.am_bf_rw : t.bf_rw);
+    return 8 * 16;
+}
+
+template <int I>
+static void launch_mfma_case(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s) {
+    constexpr MfmaTiles t = kMfmaTiles[I];
+    constexpr int ST = (int)t.steps;
+    if (a.mode == 2) {
+        if (elem == kElemF32) return launch_mfma_t<F32E, 8, 16, ST, t.am_f32_nt, true>(a, grid, s);
+        if (elem == kElemBF16)
+            return launch_mfma_t<BF16E, 8, t.am_bf_rw, ST, t.am_bf_nt, true>(a, grid, s);
+        throw Error(NNDEX_ERR_UNSUPPORTED, "no FP8 row argmax scan");
+    }
+    if (elem == kElemFP8) return launch_mfma_t<FP8E, 8, t.f8_rw, ST, t.f8_nt>(a, grid, s);
+    if (elem == kElemBF16) return launch_mfma_t<BF16E, 8, t.bf_rw, ST, t.bf_nt>(a, grid, s);
+    return launch_mfma_t<F32E, 8, 16, ST, t.f32_nt>(a, grid, s);
 }
 
 void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s) {
-    const bool bf16 = elem == kElemBF16, fp8 = elem == kElemFP8;
-    const uint32_t st = mfma_steps(a.dch);
-    if (st == 0)
+    switch (mfma_tile_index(a.dch)) {
+    case 0: return launch_mfma_case<0>(a, elem, grid, s);
+    case 1: return launch_mfma_case<1>(a, elem, grid, s);
+    case 2: return launch_mfma_case<2>(a, elem, grid, s);
+    case 3: return launch_mfma_case<3>(a, elem, grid, s);
+    case 4: return launch_mfma_case<4>(a, elem, grid, s);
+    case 5: return launch_mfma_case<5>(a, elem, grid, s);
+    case 6: return launch_mfma_case<6>(a, elem, grid, s);
+    case 7: return launch_mfma_case<7>(a, elem, grid, s);
+    default:
         throw Error(NNDEX_ERR_UNSUPPORTED, "padded dims above 512 (f32) / 1024 (bf16) / 2048 "
                                            "(fp8) are not supported by the MFMA scan");
-#define NNDEX_MFMA_CASE(STEPS, F32_NT, BF_RW, BF_NT, F8_RW, F8_NT)                    \
-    case STEPS:                                                                        \
-        if (a.mode == 2 && !bf16 && !fp8)                                              \
-            return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT, true>(a, grid, s);        \
-        if (a.mode == 2 && bf16)                                                       \
-            return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT, true>(a, grid, s);     \
-        if (a.mode == 2) break;                                                        \
-        if (fp8)                                                                       \
-            return launch_mfma_t<FP8E, 8, F8_RW, STEPS, F8_NT>(a, grid, s);            \
-        if (bf16)                                                                      \
-            return launch_mfma_t<BF16E, 8, BF_RW, STEPS, BF_NT>(a, grid, s);           \
-        return launch_mfma_t<F32E, 8, 16, STEPS, F32_NT>(a, grid, s);
-    switch (st) {
-        // (steps, F32 query tiles, BF16 rows per wave, BF16 query tiles, FP8
-        // rows per wave, FP8 query tiles): the largest tiles that fit 256
-        // VGPRs without spills
-        NNDEX_MFMA_CASE(2, 4, 32, 4, 32, 4)
-        NNDEX_MFMA_CASE(4, 4, 32, 4, 32, 2)
-        NNDEX_MFMA_CASE(6, 2, 32, 2, 32, 2)
-        NNDEX_MFMA_CASE(8, 2, 32, 2, 16, 2)
-        NNDEX_MFMA_CASE(12, 2, 16, 4, 16, 1)
-        NNDEX_MFMA_CASE(16, 2, 16, 2, 16, 1)
-        NNDEX_MFMA_CASE(24, 1, 16, 1, 16, 1)
-        NNDEX_MFMA_CASE(32, 1, 16, 1, 16, 1)
-    default:
-        break;
     }
-#undef NNDEX_MFMA_CASE
-    throw Error(NNDEX_ERR_UNSUPPORTED, "no MFMA scan variant");
 }
 
 uint32_t cw_for(uint32_t k) {

@@ -65,7 +65,8 @@ void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, int elem, 
                          const unsigned long long *xerr_sq, float *delta, hipStream_t s);
 void launch_bf16_error(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
                        hipStream_t s);
-uint32_t mfma_block_rows(int elem, uint32_t dch);
+// Rows per block of the MFMA scan (the argmax variant has its own tiles).
+uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax = false);
 void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s);
 void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
                              uint32_t kp, const float *delta, unsigned long long *thresh,

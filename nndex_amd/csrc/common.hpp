@@ -59,6 +59,12 @@ __device__ inline float4 load_nt(const float4 *p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+__device__ inline uint4 load_nt(const uint4 *p) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Read-only view in the constant address space: wave-uniform loads through it
 // become scalar (s_load) loads even when the kernel also writes memory.
 template <typename T>
@@ -88,6 +94,17 @@ __host__ __device__ inline uint32_t key_row(uint64_t key) {
 }
 __host__ __device__ inline float key_sim(uint64_t key) {
     return from_ordered(static_cast<uint32_t>(key >> 32));
+}
+// A float floor f with: make_key(s, row) >= threshold implies !(s < f), NaN
+// scores included (the test admits a superset of the passing keys): -inf for
+// thresholds at or below -inf's keys, +inf for thresholds at or above +inf's
+// keys (only +inf and NaN scores get through), else the threshold's own
+// similarity.
+__host__ __device__ inline float key_floor_sim(uint64_t threshold) {
+    const uint32_t hi = static_cast<uint32_t>(threshold >> 32);
+    if (hi <= ordered_bits(-__builtin_inff())) return -__builtin_inff();
+    if (hi >= ordered_bits(__builtin_inff())) return __builtin_inff();
+    return from_ordered(hi);
 }
 
 // Rows are stored in tiles of 64; within a tile, float4 chunk c of row r sits

@@ -405,8 +405,12 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
                 }
             }
             tk[q].compact();
-            for (uint32_t i = lane; i < a.k && (uint32_t)q < a.nvalid; i += 64) {
-                uint64_t key = i < tk[q].count ? slots[0][q][i] : 0;
+            uint32_t g = 0, e = 0;
+            if ((uint32_t)q < a.nvalid)
+                heap_tie_shift(slots[0][q], tk[q].count, a.k_out, tk[q].count >= a.rows, g, e);
+            for (uint32_t i = lane; i < a.k_out && (uint32_t)q < a.nvalid; i += 64) {
+                const uint32_t src_i = i < g ? i : i + e;
+                uint64_t key = src_i < tk[q].count ? slots[0][q][src_i] : 0;
                 a.out_idx[(size_t)q * a.out_stride + i] = key ? key_row(key) : 0;
                 a.out_sim[(size_t)q * a.out_stride + i] = key ? key_sim(key) : 0.f;
             }

@@ -513,11 +513,11 @@ size_t GpuIndex::workspace_bytes(size_t nq, size_t k, size_t ivf_nprobe) {
 }
 
 void GpuIndex::search_exact_any(Workspace &ws, const float *dq, size_t nq, size_t k,
-                                uint64_t *d_idx, float *d_sim, hipStream_t s) {
+                                uint64_t *d_idx, float *d_sim, hipStream_t s, bool heap_ties) {
     const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(k));
     const bool f32 = mfma_supported(Precision::F32), b16 = mfma_supported(Precision::BF16);
     if (nq <= 8 || !(f32 || b16) || k > 400 || qbmax == 0) {
-        search_device(ws, dq, nq, k, d_idx, d_sim, s);
+        search_device(ws, dq, nq, k, d_idx, d_sim, s, heap_ties);
         return;
     }
     // FP32 MFMA scoring up to 512 padded dims, BF16 above; the certificate
@@ -529,7 +529,8 @@ void GpuIndex::search_exact_any(Workspace &ws, const float *dq, size_t nq, size_
     NNDEX_HIP_CHECK(hipMemcpyAsync(h.data(), ws.xflags, nq, hipMemcpyDeviceToHost, s));
     NNDEX_HIP_CHECK(hipStreamSynchronize(s));
     for (size_t q = 0; q < nq; ++q)
-        if (h[q]) search_device(ws, dq + q * pdims_, 1, k, d_idx + q * k, d_sim + q * k, s);
+        if (h[q])
+            search_device(ws, dq + q * pdims_, 1, k, d_idx + q * k, d_sim + q * k, s, heap_ties);
 }
 
 static void check_rows(size_t rows) {
@@ -673,12 +674,21 @@ std::unique_ptr<Workspace> GpuIndex::make_workspace(size_t max_queries, size_t m
 }
 
 void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
-                             float *d_sim, hipStream_t s) const {
+                             float *d_sim, hipStream_t s, bool heap_ties) const {
     if (k == 0 || k > rows_)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
     if (k > ws.max_k)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k exceeds the workspace's max_k");
-    const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(k));
+    // The reference heap's choice among rows tied at the k-th score is
+    // recovered from the canonical top 2k-1 (k above ~990: canonical order).
+    size_t ksel = k;
+    if (heap_ties && k > 1 && k < rows_) {
+        const size_t k2 = 2 * k - 1 < rows_ ? 2 * k - 1 : rows_;
+        if (scan_max_qb(static_cast<uint32_t>(k2)) > 0) ksel = k2;
+    }
+    if (scan_max_qb(static_cast<uint32_t>(k)) == 0)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
+    const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(ksel));
     if (qbmax == 0)
         throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
     for (size_t q0 = 0; q0 < nq; q0 += qbmax) {
@@ -692,7 +702,8 @@ void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k
         a.q = reinterpret_cast<const float4 *>(dq + q0 * pdims_);
         a.q_stride4 = static_cast<uint32_t>(d4_);
         a.nvalid = nvalid;
-        a.k = static_cast<uint32_t>(k);
+        a.k = static_cast<uint32_t>(ksel);
+        a.k_out = static_cast<uint32_t>(k);
         a.part = ws.part;
         a.part_count = ws.part_count;
         a.theta = ws.theta;
@@ -944,7 +955,9 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     if (!ws.sub || ws.sub->max_k < nprobe)
         ws.sub = ivf.cindex->make_workspace(8, nprobe < 10 ? 10 : nprobe);
     // candidate_indices (approx.rs:110-138): top-nprobe centroids by score
-    ivf.cindex->search_exact_any(*ws.sub, dq, nq, nprobe, ws.ivf_probe, ws.ivf_probe_sim, s);
+    // the probe ranks centroids in (similarity desc, index asc) order
+    ivf.cindex->search_exact_any(*ws.sub, dq, nq, nprobe, ws.ivf_probe, ws.ivf_probe_sim, s,
+                                 false);
     IvfScanArgs a{};
     a.x = ivf.xr;
     a.d4 = static_cast<uint32_t>(d4_);

@@ -147,9 +147,12 @@ class GpuIndex {
                                                         size_t dims, bool normalize, size_t k,
                                                         bool approx);
 
-    // Enqueue an exact search of device-resident queries.
+    // Enqueue an exact search of device-resident queries.  With `heap_ties`,
+    // rows tied at the k-th score are chosen as the reference GPU path's
+    // bounded heap chooses them (src/gpu.rs:575-589); without, in (score
+    // desc, row asc) order.
     void search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
-                       float *d_sim, hipStream_t s) const;
+                       float *d_sim, hipStream_t s, bool heap_ties = true) const;
 
     // Enqueue the batched MFMA search with exact rerank and certificate
     // (no host synchronisation).  flags[q] = 1 marks a query whose result is
@@ -167,7 +170,7 @@ class GpuIndex {
     // ones the F32 MFMA path, then the scan for any uncertified query (this
     // synchronises `s` once to read the flags).
     void search_exact_any(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
-                          float *d_sim, hipStream_t s);
+                          float *d_sim, hipStream_t s, bool heap_ties = true);
     bool mfma_supported(Precision p) const;
     static size_t default_overfetch(size_t k, Precision p);
     BatchPlan plan_batch(size_t k, Precision p, size_t kp) const;

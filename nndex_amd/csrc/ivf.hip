@@ -745,7 +745,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
             m.tol = tol;
             m.recheck = recheck;
             m.recheck_count = recheck_count;
-            const uint32_t br = mfma_block_rows(bf16, m.dch);
+            const uint32_t br = mfma_block_rows(bf16, m.dch, true);
             uint32_t grid = (m.rows_padded + br - 1) / br;
             if (grid > ncu) grid = ncu;
             launch_mfma_scan(m, bf16, grid, s);

@@ -21,7 +21,9 @@ struct ScanArgs {
     const float4 *q;              // QB queries, row-major, stride q_stride4 float4
     uint32_t q_stride4;
     uint32_t nvalid;              // queries actually present (<= QB); later ones repeat the last
-    uint32_t k;                   // neighbours per query (1 <= k <= rows)
+    uint32_t k;                   // keys selected per query (1 <= k <= rows)
+    uint32_t k_out;               // neighbours written per query (<= k): when k >= 2 k_out - 1
+                                  // the reference heap's choice among ties (heap_tie_shift)
     uint64_t *part;               // [QB][grid][k] block lists
     uint32_t *part_count;         // [QB][grid] live keys per block list
     unsigned long long *theta;    // [QB] device-wide thresholds (0 between launches)

@@ -306,10 +306,36 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
         NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hlen.ptr, ex.dlen.ptr, (size_t)nq * 8,
                                        hipMemcpyDeviceToHost, ex.stream));
     NNDEX_HIP_CHECK(hipStreamSynchronize(ex.stream));
-    // scatter: each request keeps the first k of its rows' results
-    const uint64_t *hi = static_cast<const uint64_t *>(ex.hidx.ptr);
-    const float *hs = static_cast<const float *>(ex.hsim.ptr);
+    uint64_t *hi = static_cast<uint64_t *>(ex.hidx.ptr);
+    float *hs = static_cast<float *>(ex.hsim.ptr);
     const uint64_t *hl = static_cast<const uint64_t *>(ex.hlen.ptr);
+    // An exact request with k below the batch's k keeps the first k results,
+    // unless rows tie across its k-th score: the reference heap's choice
+    // among them depends on k, so such a query is searched again alone.
+    if (mode == 0) {
+        bool again = false;
+        off = 0;
+        for (auto &sg : batch) {
+            const uint32_t rk = sg.req->k;
+            for (uint32_t i = 0; i < sg.n; ++i) {
+                const size_t src = (off + i) * kstride;
+                if (rk < 2 || rk >= kstride ||
+                    __builtin_bit_cast(uint32_t, hs[src + rk - 1]) !=
+                        __builtin_bit_cast(uint32_t, hs[src + rk]))
+                    continue;
+                index_->search_device(*ex.ws, dq + (off + i) * pdims_, 1, rk, didx + src,
+                                      dsim + src, ex.stream);
+                NNDEX_HIP_CHECK(hipMemcpyAsync(hi + src, didx + src, rk * 8,
+                                               hipMemcpyDeviceToHost, ex.stream));
+                NNDEX_HIP_CHECK(hipMemcpyAsync(hs + src, dsim + src, rk * 4,
+                                               hipMemcpyDeviceToHost, ex.stream));
+                again = true;
+            }
+            off += sg.n;
+        }
+        if (again) NNDEX_HIP_CHECK(hipStreamSynchronize(ex.stream));
+    }
+    // scatter: each request keeps the first k of its rows' results
     off = 0;
     for (auto &sg : batch) {
         Request &r = *sg.req;

@@ -16,6 +16,8 @@
 
 #include <cstdint>
 
+#include "common.hpp"
+
 namespace nndex {
 
 __device__ inline void wave_lds_sync() {
@@ -105,5 +107,66 @@ struct WaveTopK {
         wave_lds_sync();
     }
 };
+
+// The reference GPU path pushes every row, in row order, into one bounded
+// heap (src/gpu.rs:575-589 with src/topk.rs:70-105): a full heap rejects a
+// score <= its minimum and otherwise evicts its smallest item, the lowest
+// score with the LOWEST row first.  Where rows tie at the k-th score s this
+// keeps a different set than the (score desc, row asc) order: with G the rows
+// above s and t* the row at which the k-th row >= s arrives, the heap holds G
+// and the tied rows up to t*, less the e lowest of those, e = |G rows after
+// t*| (each such row arrives at a full heap whose minimum is a tied row).
+//
+// Given the canonical top list L[0..n) (keys, descending) of n >= k_out + 1
+// keys, returns g (rows above s) and e, so that the reference's k_out results
+// are L[i] for i < g and L[i + e] for g <= i < k_out.  The tied rows in L are
+// in row order (key order), so the list is enough when it holds every tied
+// row (a lower key follows them, or `all_rows`: L holds every row) or at
+// least 2 k_out - 1 keys (the k_out smallest rows of G and the ties are then
+// all in L); otherwise e = 0 (canonical order).  Wave-uniform.
+// Ties are equal ordered bits, so +0 and -0 at the boundary count as distinct
+// (the reference's float test treats them as equal).
+__device__ inline void heap_tie_shift(const uint64_t *L, uint32_t n, uint32_t k_out,
+                                      bool all_rows, uint32_t &g_out, uint32_t &e_out) {
+    const uint32_t lane = __lane_id();
+    g_out = 0;
+    e_out = 0;
+    if (k_out < 2 || n <= k_out) return;
+    const uint32_t hi = static_cast<uint32_t>(L[k_out - 1] >> 32);
+    if (static_cast<uint32_t>(L[k_out] >> 32) != hi) return;  // no tie across the boundary
+    uint32_t g = 0, u = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const uint32_t h = i < n ? static_cast<uint32_t>(L[i] >> 32) : 0;
+        g += __popcll(__ballot(i < n && h > hi));
+        u += __popcll(__ballot(i < n && h >= hi));
+    }
+    if (u == n && !all_rows && n < 2 * k_out - 1) return;  // ties may continue past L
+    // t*: the k_out-th smallest row among the u rows >= s
+    uint32_t tstar = 0xFFFFFFFFu;
+    for (uint32_t i0 = 0; i0 < u; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        bool hit = false;
+        uint32_t r = 0;
+        if (i < u) {
+            r = key_row(L[i]);
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < u; ++j) rank += key_row(L[j]) < r;
+            hit = rank == k_out - 1;
+        }
+        const uint64_t m = __ballot(hit);
+        if (m) {
+            tstar = __shfl(r, __ffsll(static_cast<unsigned long long>(m)) - 1);
+            break;
+        }
+    }
+    uint32_t e = 0;
+    for (uint32_t i0 = 0; i0 < g; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        e += __popcll(__ballot(i < g && key_row(L[i]) > tstar));
+    }
+    g_out = g;
+    e_out = e;
+}
 
 }  // namespace nndex

@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
             "orc_chunk_rows": (_SZ, [_SZ, _SZ]),
             "orc_batch_chunk_rows": (_SZ, [_SZ]),
             "orc_cpu_search": (_SZ, [_F, _SZ, _SZ, _F, _SZ, _I, _U64, _F, ctypes.POINTER(_I)]),
+            "orc_gpu_exact_search": (_SZ, [_F, _SZ, _SZ, _F, _SZ, _U64, _F]),
             "orc_cpu_search_batch": (
                 None,
                 [_F, _SZ, _SZ, _F, _SZ, _SZ, _I, _U64, _F, ctypes.POINTER(_SZ), ctypes.POINTER(_I)],
@@ -259,6 +260,20 @@ def cpu_search(matrix: np.ndarray, query: np.ndarray, k: int, threads: int = 0):
     got = lib().orc_cpu_search(_fp(m), rows, dims, _fp(q), k, threads, idx.ctypes.data_as(_U64),
                                _fp(sim), ctypes.byref(st))
     return idx[:got], sim[:got], SEARCH_STRATEGIES[st.value]
+
+
+def gpu_exact_search(matrix: np.ndarray, query: np.ndarray, k: int):
+    """The GPU backend's exact search (src/gpu.rs:575-589): every row, in row
+    order, pushed into one bounded heap (its own choice among tied rows)."""
+    m = _f32(matrix)
+    q = _f32(query).ravel()
+    rows, dims = m.shape
+    kk = max(min(k, rows), 1)
+    idx = np.zeros(kk, np.uint64)
+    sim = np.zeros(kk, np.float32)
+    got = lib().orc_gpu_exact_search(_fp(m), rows, dims, _fp(q), k, idx.ctypes.data_as(_U64),
+                                     _fp(sim))
+    return idx[:got], sim[:got]
 
 
 def cpu_search_batch(matrix: np.ndarray, queries: np.ndarray, k: int, threads: int = 0):

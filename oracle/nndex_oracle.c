@@ -15,6 +15,9 @@
  *   - TopK accumulator and topk_from_scores: topk.rs:44-210, pinned by the 20
  *     KATs of topk.rs:216-477 restated in tests, and by brute-force sorting.
  *   - strategy selection: profiles.rs:72-185, pinned by profiles.rs:194-343.
+ *   - orc_gpu_exact_search: gpu.rs:575-589 / :758-776 (one accumulator, rows
+ *     pushed in order); its choice among rows tied at the k-th score pinned by
+ *     a closed-form restatement of the heap and by a hand-worked case.
  *   - orc_sgemm_scores: restates matrixmultiply's f32 kernel (sequential-k FMA
  *     within KC=256 blocks, block partials added into C).  Bitwise parity
  *     UNPINNED (matrixmultiply is not vendored; the reference's macOS build may
@@ -515,6 +518,20 @@ size_t orc_cpu_search(const float *matrix, size_t rows, size_t dims, const float
     size_t capped = k < rows ? k : rows;
     orc_topk *acc =
         search_with_profile(&c, query, capped, orc_profile_cutoff(rows, dims, 1), strategy_out);
+    return orc_topk_into_sorted(acc, idx, sim);
+}
+
+/* GPU backend's exact search (gpu.rs:575-589 single query, :758-776 per query
+ * of a batch): every row's score, in row order, pushed into one accumulator.
+ * Where rows tie at the k-th score this keeps a different set from the
+ * (score desc, row asc) order; the set is pinned by a closed-form restatement
+ * of the heap's behaviour in tests/test_oracle.py. */
+size_t orc_gpu_exact_search(const float *matrix, size_t rows, size_t dims, const float *query,
+                            size_t k, uint64_t *idx, float *sim) {
+    size_t capped = k < rows ? k : rows;
+    orc_topk *acc = orc_topk_new(capped);
+    for (size_t r = 0; r < rows; ++r)
+        orc_topk_push(acc, r, orc_dot_product(matrix + r * dims, query, dims));
     return orc_topk_into_sorted(acc, idx, sim);
 }
 

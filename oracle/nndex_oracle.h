@@ -87,6 +87,10 @@ size_t orc_cpu_search(const float *matrix, size_t rows, size_t dims, const float
 void orc_cpu_search_batch(const float *matrix, size_t rows, size_t dims, const float *queries,
                           size_t query_rows, size_t k, int threads, uint64_t *idx, float *sim,
                           size_t *out_lens, int *strategy_out);
+/* GPU backend's exact search (src/gpu.rs:575-589): rows pushed in order into
+ * one accumulator. */
+size_t orc_gpu_exact_search(const float *matrix, size_t rows, size_t dims, const float *query,
+                            size_t k, uint64_t *idx, float *sim);
 /* sgemm restatement: scores[q*rows + r] = Q[q,:] . X[r,:] */
 void orc_sgemm_scores(const float *queries, size_t query_rows, const float *matrix, size_t rows,
                       size_t dims, int threads, float *scores);

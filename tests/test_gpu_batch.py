@@ -158,3 +158,41 @@ def test_batch_position_independence_mfma():
     assert np.array_equal(i1[perm], i2) and np.array_equal(s1[perm], s2)
     i3, s3 = exact_device(b, ds, qs[:5], 10)  # scan path for the first five
     assert np.array_equal(i1[:5], i3) and np.array_equal(s1[:5], s3)
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_batch_duplicate_rows_ties_at_threshold(orc, precision):
+    """64 distinct rows repeated: every sample threshold sits on a score that
+    hundreds of rows share, so candidates tie at the threshold (decided by the
+    row part of the key), the per-wave candidate queues fill and flush many
+    times per row tile and the per-query lists overflow.  Certified results
+    equal the exact search, every returned similarity is exact, and the exact
+    search (certificate + scan fallback, device and frontend) equals the
+    oracle's restatement of the reference GPU path bit for bit."""
+    _case("duplicates", precision=precision)
+    rng = np.random.default_rng(77)
+    distinct = rng.normal(size=(64, 128)).astype(np.float32)
+    distinct /= np.linalg.norm(distinct, axis=1, keepdims=True)
+    base = np.ascontiguousarray(np.tile(distinct, (625, 1)))  # 40,000 rows
+    qs = rng.normal(size=(48, 128)).astype(np.float32)
+    qs /= np.linalg.norm(qs, axis=1, keepdims=True)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 48, 74)
+    gi, gs, fl = batch_device(b, ds, qs, 10, precision)
+    ei, es = exact_device(b, ds, qs, 10)
+    for i in range(0, 48, 5):
+        oidx, osim = orc.gpu_exact_search(base, qs[i], 10)
+        assert ei[i].tolist() == oidx.tolist() and es[i].tolist() == osim.tolist(), i
+    for i in range(48):
+        if not fl[i]:
+            assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist(), i
+        for j in range(10):
+            assert gs[i, j] == np.float32(orc.dot_product(base[int(gi[i, j])], qs[i])), (i, j)
+    # through the frontend (uncertified queries are rescanned exactly)
+    index = nndex_amd.NNdex(base, normalized=False, backend="gpu", enable_cache=False)
+    bidx, bsim = index.search(qs, k=10)
+    om = orc.normalize_rows(base, base.shape[0], 128)
+    oq = orc.normalize_rows(qs, 48, 128)
+    for i in range(48):
+        oidx, osim = orc.gpu_exact_search(om, oq[i], 10)
+        assert bidx[i].tolist() == oidx.tolist() and bsim[i].tolist() == osim.tolist(), i

@@ -125,6 +125,39 @@ def test_ties_resolve_to_lower_index(orc):
     assert len(set(sim.tolist())) == 1
 
 
+def test_ties_follow_the_reference_heap(orc):
+    """Rows tied at the k-th score, with higher-scoring rows placed after them:
+    the reference GPU path's heap (rows pushed in order, lowest row evicted
+    first among equal minima) keeps later tied rows than the (score desc, row
+    asc) order would.  Single queries, batches (the certified MFMA path hands
+    such queries to the scan) and the device entry points must all agree with
+    the oracle's restatement of that path."""
+    _print_case("heap_ties")
+    rng = np.random.default_rng(11)
+    pool = rng.normal(size=(6, 32)).astype(np.float32)
+    rows = np.concatenate([rng.integers(1, 6, size=3000), np.zeros(20, np.int64)])
+    m = np.ascontiguousarray(pool[rows])
+    qs = np.ascontiguousarray(pool[rng.integers(0, 6, size=12)] +
+                              0.2 * rng.normal(size=(12, 32)).astype(np.float32))
+    om = orc.normalize_rows(m, m.shape[0], 32)
+    oq = orc.normalize_rows(qs, 12, 32)
+    index = nndex_amd.NNdex(m, normalized=False, backend="gpu", enable_cache=False)
+    shifted = 0
+    for k in (2, 25, 60):
+        bidx, bsim = index.search(qs, k=k)
+        for i in range(12):
+            # single queries are normalised as vectors, batches as rows
+            sidx, ssim = index.search(qs[i], k=k)
+            o1idx, o1sim = orc.gpu_exact_search(om, orc.normalize_vector(qs[i]), k)
+            assert sidx.tolist() == o1idx.tolist() and ssim.tolist() == o1sim.tolist(), (k, i)
+            oidx, osim = orc.gpu_exact_search(om, oq[i], k)
+            assert bidx[i].tolist() == oidx.tolist() and bsim[i].tolist() == osim.tolist(), (k, i)
+            sc = [orc.dot_product(om[r], oq[i]) for r in range(om.shape[0])]
+            canon = sorted(range(om.shape[0]), key=lambda r: (-sc[r], r))[:k]
+            shifted += canon != oidx.tolist()
+    assert shifted > 0  # the case exercises the heap's own choice
+
+
 def test_all_negative_scores_and_k_capping():
     """src/lib.rs:1466-1506 and :1374-1425."""
     _print_case("negatives")

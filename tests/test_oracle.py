@@ -230,6 +230,56 @@ def test_topk_heap_evicts_lowest_index_among_equal_minima(orc):
     assert _acc(orc, 2, [(0, .5), (1, .5), (2, .7)]).into_sorted()[0].tolist() == [2, 1]
 
 
+def _heap_rule(scores, k):
+    """Closed form of a bounded heap fed every row in row order (push rejects
+    a score <= the minimum of a full heap; eviction takes the lowest score,
+    lowest row first): G = rows above the k-th score s, t* = the k-th smallest
+    row among the rows >= s, P = rows tied at s up to t*; the heap keeps G and
+    P less its e lowest rows, e = |G rows after t*|."""
+    n = len(scores)
+    k = min(k, n)
+    canon = sorted(range(n), key=lambda r: (-scores[r], r))
+    if k <= 1 or n == k:
+        return canon[:k]
+    s = scores[canon[k - 1]]
+    g_rows = [r for r in range(n) if scores[r] > s]
+    t_rows = [r for r in range(n) if scores[r] == s]
+    tstar = sorted(g_rows + t_rows)[k - 1]
+    p_rows = [r for r in t_rows if r <= tstar]
+    e = sum(1 for r in g_rows if r > tstar)
+    return sorted(g_rows + p_rows[e:], key=lambda r: (-scores[r], r))
+
+
+def test_gpu_exact_search_hand_worked_tie(orc):
+    # scores 0.5, 0.5, 0.9 (exact in f32): the third row evicts row 0, the
+    # lowest row among the two minima, so the heap answers [2, 1], not [2, 0]
+    m = np.array([[0.5, 0.75 ** 0.5], [0.5, 0.75 ** 0.5], [0.9, 0.19 ** 0.5]], np.float32)
+    idx, sim = orc.gpu_exact_search(m, np.array([1.0, 0.0], np.float32), 2)
+    assert idx.tolist() == [2, 1] and sim.tolist() == [np.float32(0.9), np.float32(0.5)]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_gpu_exact_search_matches_closed_form_heap_rule(orc, seed):
+    """Rows drawn from a small pool (exact ties everywhere): the oracle's
+    GPU-path search equals the closed form for every k, and differs from the
+    (score desc, row asc) order somewhere."""
+    rng = np.random.default_rng(seed)
+    pool = rng.normal(size=(5, 16)).astype(np.float32)
+    pool /= np.linalg.norm(pool, axis=1, keepdims=True)
+    m = np.ascontiguousarray(pool[rng.integers(0, 5, size=60)])
+    q = pool[0] + 0.3 * rng.normal(size=16).astype(np.float32)
+    q = (q / np.linalg.norm(q)).astype(np.float32)
+    scores = [orc.dot_product(m[r], q) for r in range(60)]
+    differs = False
+    for k in range(1, 61):
+        idx, sim = orc.gpu_exact_search(m, q, k)
+        want = _heap_rule(scores, k)
+        assert idx.tolist() == want, k
+        assert sim.tolist() == [np.float32(scores[r]) for r in want]
+        differs |= want != sorted(range(60), key=lambda r: (-scores[r], r))[:k]
+    assert differs
+
+
 def test_topk_fewer_than_k_and_k1(orc):
     assert _acc(orc, 10, [(0, .3), (1, .7)]).into_sorted()[0].tolist() == [1, 0]
     idx, sim = _acc(orc, 1, [(0, .1), (1, .9), (2, .5)]).into_sorted()
