@@ -59,12 +59,6 @@ __device__ inline float4 load_nt(const float4 *p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-__device__ inline uint4 load_nt(const uint4 *p) {
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
-    return make_uint4(v.x, v.y, v.z, v.w);
-}
-
 // Read-only view in the constant address space: wave-uniform loads through it
 // become scalar (s_load) loads even when the kernel also writes memory.
 template <typename T>

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     // publish all their keys (the theta slots stay 0) to the final merge.
     WaveTopK tk[QB];
 #pragma unroll
-    for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, nullptr);
+    for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, nullptr, a.k_out);
 
     // U float4 chunks per load group (two groups in flight); NT selects
     // streaming (non-temporal) loads at compile time so the loop is straight
@@ -343,9 +343,15 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
                 const uint32_t i = i0 + lane;
                 live += __popcll(__ballot(i < tk[q].count && slots[0][q][i] >= th));
             }
-            uint64_t *dst = a.part + ((size_t)q * gridDim.x + blockIdx.x) * a.k;
+            // one dense list per query: the block reserves its range with
+            // one atomic, so the final merge reads no empty slots
+            uint32_t base = 0;
+            if (lane == 0 && live)
+                base = __hip_atomic_fetch_add(a.part_count + q, live, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            base = __shfl(base, 0);
+            uint64_t *dst = a.part + (size_t)q * gridDim.x * a.k + base;
             for (uint32_t i = lane; i < live; i += 64) dst[i] = slots[0][q][i];
-            if (lane == 0) a.part_count[(size_t)q * gridDim.x + blockIdx.x] = live;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t ticket = 0;
@@ -359,22 +365,17 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     if (!is_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    // Final merge of gridDim.x block lists per query, keys >= final threshold:
-    // the block counts go to LDS in one round trip, then every (block, slot)
-    // key is loaded in independent batches of 8 per lane before it is offered.
-    __shared__ uint32_t bcnt[2048];  // grids are capped at 8 blocks per CU
+    // Final merge of the published keys per query (>= final threshold), in
+    // independent batches of 8 loads per lane before they are offered.
     const uint32_t nb = gridDim.x;
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
         const unsigned long long theta =
             __hip_atomic_load(a.theta + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tk[q].init(slots[wid][q], CW, a.k, nullptr);
-        __syncthreads();  // bcnt reuse across queries
-        for (uint32_t b = threadIdx.x; b < nb && b < 2048; b += 256)
-            bcnt[b] = a.part_count[(size_t)q * nb + b];
-        __syncthreads();
+        tk[q].init(slots[wid][q], CW, a.k, nullptr, a.k_out);
         const uint64_t *src = a.part + (size_t)q * nb * a.k;
-        const uint32_t total = nb * a.k;
+        const uint32_t total =
+            __hip_atomic_load(a.part_count + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         constexpr int KB = 8;
         for (uint32_t j0 = wid * 64 * KB; j0 < total; j0 += 256 * KB) {
             uint64_t key[KB];
@@ -382,8 +383,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
 #pragma unroll
             for (int u = 0; u < KB; ++u) {
                 const uint32_t j = j0 + u * 64 + lane;
-                const uint32_t b = j / a.k, i = j % a.k;
-                ok[u] = j < total && i < bcnt[b];
+                ok[u] = j < total;
                 key[u] = ok[u] ? src[j] : 0;
             }
 #pragma unroll
@@ -407,15 +407,19 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
             tk[q].compact();
             uint32_t g = 0, e = 0;
             if ((uint32_t)q < a.nvalid)
-                heap_tie_shift(slots[0][q], tk[q].count, a.k_out, tk[q].count >= a.rows, g, e);
+                heap_tie_shift(slots[0][q], tk[q].count, a.k_out,
+                               a.k + 1 >= 2 * a.k_out || a.k >= a.rows, g, e);
             for (uint32_t i = lane; i < a.k_out && (uint32_t)q < a.nvalid; i += 64) {
                 const uint32_t src_i = i < g ? i : i + e;
                 uint64_t key = src_i < tk[q].count ? slots[0][q][src_i] : 0;
                 a.out_idx[(size_t)q * a.out_stride + i] = key ? key_row(key) : 0;
                 a.out_sim[(size_t)q * a.out_stride + i] = key ? key_sim(key) : 0.f;
             }
-            if (lane == 0)
+            if (lane == 0) {
                 __hip_atomic_store(a.theta + q, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.part_count + q, 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         if (lane == 0) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -668,6 +668,7 @@ std::unique_ptr<Workspace> GpuIndex::make_workspace(size_t max_queries, size_t m
     NNDEX_HIP_CHECK(hipMalloc(&ws->part_count, 8 * gmax * sizeof(uint32_t)));
     NNDEX_HIP_CHECK(hipMalloc(&ws->theta, 8 * sizeof(unsigned long long)));
     NNDEX_HIP_CHECK(hipMalloc(&ws->ticket, sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMemset(ws->part_count, 0, 8 * gmax * sizeof(uint32_t)));
     NNDEX_HIP_CHECK(hipMemset(ws->theta, 0, 8 * sizeof(unsigned long long)));
     NNDEX_HIP_CHECK(hipMemset(ws->ticket, 0, sizeof(uint32_t)));
     return ws;

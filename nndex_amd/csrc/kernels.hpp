@@ -24,8 +24,8 @@ struct ScanArgs {
     uint32_t k;                   // keys selected per query (1 <= k <= rows)
     uint32_t k_out;               // neighbours written per query (<= k): when k >= 2 k_out - 1
                                   // the reference heap's choice among ties (heap_tie_shift)
-    uint64_t *part;               // [QB][grid][k] block lists
-    uint32_t *part_count;         // [QB][grid] live keys per block list
+    uint64_t *part;               // [QB][grid * k] published keys, one dense list per query
+    uint32_t *part_count;         // [QB] keys published per query (0 between launches)
     unsigned long long *theta;    // [QB] device-wide thresholds (0 between launches)
     uint32_t *ticket;             // block-completion counter (0 between launches)
     uint64_t *out_idx;            // [QB][out_stride]

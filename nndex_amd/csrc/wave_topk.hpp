@@ -30,14 +30,20 @@ struct WaveTopK {
     uint64_t *buf;    // LDS slots
     uint32_t cap;     // power of two
     uint32_t k;       // keys to keep
+    uint32_t rank;    // keys scoring below the rank-th key are dropped (rank <= k)
     uint32_t count;   // wave-uniform
     uint64_t thresh;  // wave-uniform: keys must be > thresh
     unsigned long long *global_thresh;  // device-wide k-th key, or nullptr
 
-    __device__ void init(uint64_t *b, uint32_t c, uint32_t kk, unsigned long long *g) {
+    // rank < k keeps, besides the best `rank` keys, up to k - rank more keys
+    // that tie the rank-th key's score (lowest rows first): the keys the
+    // reference heap's tie rule can need.
+    __device__ void init(uint64_t *b, uint32_t c, uint32_t kk, unsigned long long *g,
+                         uint32_t rk = 0) {
         buf = b;
         cap = c;
         k = kk;
+        rank = rk && rk < kk ? rk : kk;
         count = 0;
         thresh = 0;
         global_thresh = g;
@@ -69,10 +75,24 @@ struct WaveTopK {
         }
     }
 
-    // Sort, keep the best k, raise the threshold.
+    // Sort, keep the best k (and only keys scoring at least the rank-th
+    // key's score), raise the threshold.
     __device__ void compact() {
         sort_desc();
         if (count > k) count = k;
+        if (rank < k && count >= rank) {
+            // floor: below every key with the rank-th key's score
+            const uint64_t floor = (buf[rank - 1] & 0xFFFFFFFF00000000ull) - 1;
+            for (uint32_t i0 = rank; i0 < count; i0 += 64) {
+                const uint32_t i = i0 + __lane_id();
+                const uint64_t m = __ballot(i < count && buf[i] <= floor);
+                if (m) {
+                    count = i0 + __ffsll(static_cast<unsigned long long>(m)) - 1;
+                    break;
+                }
+            }
+            if (floor > thresh) thresh = floor;
+        }
         if (k > 0 && count == k) {
             uint64_t kth = buf[k - 1];
             if (kth > thresh) thresh = kth;
@@ -117,21 +137,21 @@ struct WaveTopK {
 // and the tied rows up to t*, less the e lowest of those, e = |G rows after
 // t*| (each such row arrives at a full heap whose minimum is a tied row).
 //
-// Given the canonical top list L[0..n) (keys, descending) of n >= k_out + 1
-// keys, returns g (rows above s) and e, so that the reference's k_out results
-// are L[i] for i < g and L[i + e] for g <= i < k_out.  The tied rows in L are
-// in row order (key order), so the list is enough when it holds every tied
-// row (a lower key follows them, or `all_rows`: L holds every row) or at
-// least 2 k_out - 1 keys (the k_out smallest rows of G and the ties are then
-// all in L); otherwise e = 0 (canonical order).  Wave-uniform.
+// Given the canonical top list L[0..n) (keys, descending), returns g (rows
+// above s) and e, so that the reference's k_out results are L[i] for i < g
+// and L[i + e] for g <= i < k_out.  The tied rows in L are in row order (key
+// order); `complete` says L holds G and the k_out lowest tied rows (or every
+// tied row): true for a canonical top-(2 k_out - 1) selection, possibly
+// trimmed of keys scoring below s (WaveTopK's rank), or for one covering
+// every row.  Otherwise e = 0 (canonical order).  Wave-uniform.
 // Ties are equal ordered bits, so +0 and -0 at the boundary count as distinct
 // (the reference's float test treats them as equal).
 __device__ inline void heap_tie_shift(const uint64_t *L, uint32_t n, uint32_t k_out,
-                                      bool all_rows, uint32_t &g_out, uint32_t &e_out) {
+                                      bool complete, uint32_t &g_out, uint32_t &e_out) {
     const uint32_t lane = __lane_id();
     g_out = 0;
     e_out = 0;
-    if (k_out < 2 || n <= k_out) return;
+    if (k_out < 2 || n <= k_out || !complete) return;
     const uint32_t hi = static_cast<uint32_t>(L[k_out - 1] >> 32);
     if (static_cast<uint32_t>(L[k_out] >> 32) != hi) return;  // no tie across the boundary
     uint32_t g = 0, u = 0;
@@ -141,7 +161,6 @@ __device__ inline void heap_tie_shift(const uint64_t *L, uint32_t n, uint32_t k_
         g += __popcll(__ballot(i < n && h > hi));
         u += __popcll(__ballot(i < n && h >= hi));
     }
-    if (u == n && !all_rows && n < 2 * k_out - 1) return;  // ties may continue past L
     // t*: the k_out-th smallest row among the u rows >= s
     uint32_t tstar = 0xFFFFFFFFu;
     for (uint32_t i0 = 0; i0 < u; i0 += 64) {
