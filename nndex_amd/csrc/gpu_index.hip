@@ -86,7 +86,7 @@ void Workspace::reserve_ivf(size_t pairs, size_t nlist, size_t part_elems, size_
     if (pairs > ivf_pairs) {
         regrow(ivf_probe, pairs);
         regrow(ivf_probe_sim, pairs);
-        regrow(ivf_pair_u32, 4 * pairs);
+        regrow(ivf_pair_u32, 5 * pairs);
         ivf_pairs = pairs;
     }
     if (nlist > ivf_lists) {
@@ -477,8 +477,9 @@ void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hi
                     for (size_t kk : {(size_t)1, (size_t)10, k}) {
                         if (kk > k) continue;
                         uint32_t qb, nchunk;
-                        ivf_scan_shape(n, p, ivf.nlist, rows_, kk, qb, nchunk);
-                        const size_t e = n * p * nchunk * kk;
+                        const size_t ks = ivf_select_k(kk);
+                        ivf_scan_shape(n, p, ivf.nlist, rows_, ks, qb, nchunk);
+                        const size_t e = n * p * nchunk * ks;
                         part = e > part ? e : part;
                     }
             ws.reserve_ivf(nq * np, ivf.nlist, part, ivf_sort_bytes(nq * np, ivf.nlist));
@@ -950,9 +951,10 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     if (nprobe == 0) nprobe = ivf_choose_nprobe(ivf.nlist, k, rows_);
     if (nprobe > ivf.nlist) nprobe = ivf.nlist;
     uint32_t qb, nchunk;
-    ivf_scan_shape(nq, nprobe, ivf.nlist, rows_, k, qb, nchunk);
+    const uint32_t kk = ivf_select_k(k);
+    ivf_scan_shape(nq, nprobe, ivf.nlist, rows_, kk, qb, nchunk);
     const size_t pairs = nq * nprobe;
-    ws.reserve_ivf(pairs, ivf.nlist, pairs * nchunk * k, ivf_sort_bytes(pairs, ivf.nlist));
+    ws.reserve_ivf(pairs, ivf.nlist, pairs * nchunk * kk, ivf_sort_bytes(pairs, ivf.nlist));
     if (!ws.sub || ws.sub->max_k < nprobe)
         ws.sub = ivf.cindex->make_workspace(8, nprobe < 10 ? 10 : nprobe);
     // candidate_indices (approx.rs:110-138): top-nprobe centroids by score
@@ -968,6 +970,7 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     a.nprobe = static_cast<uint32_t>(nprobe);
     a.nlist = ivf.nlist;
     a.k = static_cast<uint32_t>(k);
+    a.kk = kk;
     a.qb = qb;
     a.nchunk = nchunk;
     a.offsets = ivf.offsets;
@@ -976,6 +979,7 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     a.pair_val = ws.ivf_pair_u32 + pairs;
     a.sorted_key = ws.ivf_pair_u32 + 2 * pairs;
     a.sorted_pair = ws.ivf_pair_u32 + 3 * pairs;
+    a.pair_base = ws.ivf_pair_u32 + 4 * pairs;
     const size_t L1 = ivf.nlist + 1;
     a.pair_cnt = ws.ivf_list_u32;
     a.pair_off = ws.ivf_list_u32 + L1;

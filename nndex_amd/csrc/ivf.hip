@@ -463,6 +463,21 @@ __global__ void pair_prep_kernel(const uint64_t *__restrict__ probe, uint32_t pa
     }
 }
 
+// Arrival base of each (query, probe): the rows of the query's lists probed
+// before it (the reference pushes rows list by list in probe order).
+__global__ void pair_base_kernel(const uint64_t *__restrict__ probe, uint32_t nq, uint32_t nprobe,
+                                 const uint32_t *__restrict__ offsets,
+                                 uint32_t *__restrict__ base) {
+    for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < nq; q += gridDim.x * 256) {
+        uint32_t acc = 0;
+        for (uint32_t p = 0; p < nprobe; ++p) {
+            base[(size_t)q * nprobe + p] = acc;
+            const uint32_t c = (uint32_t)probe[(size_t)q * nprobe + p];
+            acc += offsets[c + 1] - offsets[c];
+        }
+    }
+}
+
 // Wavefronts per list: ceil(pairs of the list / qb).
 __global__ void group_count_kernel(const uint32_t *__restrict__ pair_cnt, uint32_t nlist,
                                    uint32_t qb, uint32_t *__restrict__ grp_cnt) {
@@ -474,7 +489,9 @@ __global__ void group_count_kernel(const uint32_t *__restrict__ pair_cnt, uint32
 // 64-row tile scores row r against the group's QB queries in the reference
 // dot order (four-lane FMA chains, (l0 + l1) + (l2 + l3)); the queries come
 // through the scalar cache; each query keeps its own wave top-k in LDS and
-// writes its best k keys (original row ids) for this chunk.
+// writes its best kk keys for this chunk.  A key's row part is the row's
+// arrival index in the query's probe sequence (list by list in probe order,
+// reordered position within a list), the order the reference pushes rows in.
 template <int QB, int CW>
 __global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
     __shared__ uint64_t slots[4][QB][CW];
@@ -511,7 +528,10 @@ __global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
     const uint32_t r1 = l0 + ((chunk + 1) * per < len ? (chunk + 1) * per : len);
     WaveTopK tk[QB];
 #pragma unroll
-    for (int j = 0; j < QB; ++j) tk[j].init(slots[wid][j], CW, a.k, nullptr);
+    for (int j = 0; j < QB; ++j) tk[j].init(slots[wid][j], CW, a.kk, nullptr, a.k);
+    uint32_t abase[QB];
+#pragma unroll
+    for (int j = 0; j < QB; ++j) abase[j] = a.pair_base[pid[j]] - l0;
     for (uint32_t t = r0 / 64; t * 64 < r1; ++t) {
         const uint32_t row = t * 64 + lane;
         const bool valid = row >= r0 && row < r1;
@@ -531,33 +551,37 @@ __global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
                 acc[j][3] = __builtin_fmaf(v.w, qv.w, acc[j][3]);
             }
         }
-        const uint32_t id = valid ? a.orig[row] : 0;
 #pragma unroll
         for (int j = 0; j < QB; ++j) {
             if ((uint32_t)j >= pn) break;
             const float sc = (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
-            tk[j].offer(valid ? make_key(sc, id) : 0, valid);
+            tk[j].offer(valid ? make_key(sc, abase[j] + row) : 0, valid);
         }
     }
 #pragma unroll
     for (int j = 0; j < QB; ++j) {
         if ((uint32_t)j >= pn) break;
         tk[j].compact();
-        uint64_t *dst = a.part + ((size_t)pid[j] * a.nchunk + chunk) * a.k;
-        for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk[j].count ? slots[wid][j][i] : 0;
+        uint64_t *dst = a.part + ((size_t)pid[j] * a.nchunk + chunk) * a.kk;
+        for (uint32_t i = lane; i < a.kk; i += 64)
+            dst[i] = i < tk[j].count ? slots[wid][j][i] : 0;
     }
 }
 
 // One block per query: its nprobe x nchunk partial lists merged into the
-// final top-k and the number found.
+// final top-k and the number found.  The merged list is in (score desc,
+// arrival asc) order; the reference heap's choice among rows tied at the k-th
+// score follows from it (heap_tie_shift over arrival indices, then the heap
+// evicts the lowest original rows first), and the results are written in
+// (score desc, original row asc) order as the reference sorts them.
 template <int CW>
 __global__ __launch_bounds__(256) void ivf_merge_kernel(IvfScanArgs a) {
     __shared__ uint64_t slots[4][CW];
     __shared__ uint32_t cnt[4];
     const uint32_t q = blockIdx.x, lane = __lane_id(), wid = threadIdx.x >> 6;
     WaveTopK tk;
-    tk.init(slots[wid], CW, a.k, nullptr);
-    const uint32_t n = a.nprobe * a.nchunk * a.k;
+    tk.init(slots[wid], CW, a.kk, nullptr, a.k);
+    const uint32_t n = a.nprobe * a.nchunk * a.kk;
     const uint64_t *src = a.part + (size_t)q * n;
     for (uint32_t i0 = wid * 64; i0 < n; i0 += 256) {
         const uint32_t i = i0 + lane;
@@ -574,12 +598,50 @@ __global__ __launch_bounds__(256) void ivf_merge_kernel(IvfScanArgs a) {
             tk.offer(i < cnt[w] ? slots[w][i] : 0, i < cnt[w]);
         }
     tk.compact();
+    const uint64_t *L = slots[0];
+    const uint32_t m = tk.count < a.k ? tk.count : a.k;  // results
+    uint32_t g = 0, e = 0;
+    heap_tie_shift(L, tk.count, a.k, a.kk + 1 >= 2 * a.k, g, e);
+    // window L[0, m + e): rows above the k-th score, then the tied rows the
+    // heap saw before its k-th row >= that score; e of those are evicted
+    const uint32_t w = m + e;
+    uint32_t *ids = reinterpret_cast<uint32_t *>(slots[1]);  // waves 1-3 are done
+    const uint32_t *pb = a.pair_base + (size_t)q * a.nprobe;
+    for (uint32_t i = lane; i < w; i += 64) {
+        const uint32_t arr = key_row(L[i]);
+        uint32_t lo = 0, hi = a.nprobe;  // last probe whose base <= arr
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (pb[mid] <= arr)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t c = (uint32_t)a.probe[(size_t)q * a.nprobe + lo];
+        ids[i] = a.orig[a.offsets[c] + (arr - pb[lo])];
+    }
+    wave_lds_sync();
+    uint64_t *res = slots[2];
+    for (uint32_t i = lane; i < w; i += 64) {
+        bool keep = true;
+        if (i >= g && e) {
+            uint32_t rank = 0;  // among the tied window rows, by original row
+            for (uint32_t j = g; j < w; ++j) rank += ids[j] < ids[i];
+            keep = rank >= e;
+        }
+        res[i] = keep ? ((L[i] & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - ids[i])) : 0;
+    }
+    wave_lds_sync();
+    WaveTopK out;
+    out.init(res, CW, w, nullptr);
+    out.count = w;
+    out.sort_desc();
     for (uint32_t i = lane; i < a.k; i += 64) {
-        const uint64_t key = i < tk.count ? slots[0][i] : 0;
+        const uint64_t key = i < m ? res[i] : 0;
         a.out_idx[(size_t)q * a.k + i] = key ? key_row(key) : 0;
         a.out_sim[(size_t)q * a.k + i] = key ? key_sim(key) : 0.f;
     }
-    if (lane == 0) a.out_len[q] = tk.count;
+    if (lane == 0) a.out_len[q] = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -803,6 +865,10 @@ size_t ivf_sort_bytes(size_t pairs, size_t nlist) {
     return bytes;
 }
 
+uint32_t ivf_select_k(size_t k) {
+    return 2 * k - 1 <= 960 ? static_cast<uint32_t>(2 * k - 1) : static_cast<uint32_t>(k);
+}
+
 void ivf_scan_shape(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k, uint32_t &qb,
                     uint32_t &nchunk) {
     const size_t pairs = nq * nprobe;
@@ -826,6 +892,7 @@ void ivf_scan_shape(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t 
 }
 
 size_t ivf_scan_scratch_bytes(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k) {
+    k = ivf_select_k(k);
     uint32_t qb, nchunk;
     size_t worst = 0;
     // the part buffer is largest at the most chunks: try every batch size's
@@ -839,7 +906,7 @@ size_t ivf_scan_scratch_bytes(size_t nq, size_t nprobe, size_t nlist, size_t row
     const size_t b = nq * nprobe * nchunk * k * 8;
     worst = b > worst ? b : worst;
     const size_t pairs = nq * nprobe;
-    return worst + pairs * 4 * 4 + (nlist + 1) * 4 * 4 + ivf_sort_bytes(pairs, nlist) + 1024;
+    return worst + pairs * 5 * 4 + (nlist + 1) * 4 * 4 + ivf_sort_bytes(pairs, nlist) + 1024;
 }
 
 void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
@@ -849,6 +916,8 @@ void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
     if (grid > 1024) grid = 1024;
     hipLaunchKernelGGL(pair_prep_kernel, dim3(grid), dim3(256), 0, s, a.probe, pairs, a.pair_key,
                        a.pair_val);
+    hipLaunchKernelGGL(pair_base_kernel, dim3((a.nq + 255) / 256), dim3(256), 0, s, a.probe, a.nq,
+                       a.nprobe, a.offsets, a.pair_base);
     size_t bytes = a.sort_bytes;
     NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(a.sort_tmp, bytes, a.pair_key, a.sorted_key,
                                               a.pair_val, a.sorted_pair, pairs, 0,
@@ -867,7 +936,7 @@ void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
     const size_t gmax = (pairs + a.qb - 1) / a.qb + (a.nlist < pairs ? a.nlist : pairs);
     const size_t waves = gmax * a.nchunk;
     const uint32_t sgrid = static_cast<uint32_t>((waves + 3) / 4);
-    const uint32_t cw = ivf_cw(a.k);
+    const uint32_t cw = ivf_cw(a.kk);
 #define NNDEX_LIST_CASE(QBV, C)                                                          \
     if (a.qb == QBV && cw == C) {                                                        \
         hipLaunchKernelGGL((ivf_list_scan_kernel<QBV, C>), dim3(sgrid), dim3(256), 0, s, a); \

@@ -45,15 +45,17 @@ struct IvfScanArgs {
     const float *q;             // [nq][d4 * 4] normalised padded queries
     const uint64_t *probe;      // [nq][nprobe] probed cluster ids
     uint32_t nq, nprobe, nlist, k;
+    uint32_t kk;                // keys kept per partial list (ivf_select_k(k))
     uint32_t qb;                // pairs per wavefront (1, 4 or 8)
     uint32_t nchunk;            // row chunks per list
     const uint32_t *offsets;    // [nlist + 1] list offsets in the reordered rows
     const uint32_t *orig;       // [rows] original row of each reordered position
     // scratch (IvfScratch::reserve sizes them)
     uint32_t *pair_key, *pair_val, *sorted_key, *sorted_pair;  // [nq * nprobe]
+    uint32_t *pair_base;        // [nq * nprobe] rows of the query's earlier probed lists
     uint32_t *pair_cnt, *pair_off;  // [nlist], [nlist + 1]
     uint32_t *grp_cnt, *grp_off;    // [nlist], [nlist + 1]
-    uint64_t *part;             // [nq * nprobe][nchunk][k]
+    uint64_t *part;             // [nq * nprobe][nchunk][kk]
     void *sort_tmp;
     size_t sort_bytes;
     uint64_t *out_idx;          // [nq][k]
@@ -61,6 +63,10 @@ struct IvfScanArgs {
     uint64_t *out_len;          // [nq]
 };
 
+// Keys the list scan keeps per (query, list, chunk) for k results: 2k - 1, so
+// the merge can apply the reference heap's choice among tied rows (k <= 480),
+// else k.
+uint32_t ivf_select_k(size_t k);
 // Launch shape of the list scan for a batch: pairs per wavefront and row
 // chunks per list (enough wavefronts to fill the device, a bounded merge).
 void ivf_scan_shape(size_t nq, size_t nprobe, size_t nlist, size_t rows, size_t k, uint32_t &qb,

@@ -309,10 +309,10 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
     uint64_t *hi = static_cast<uint64_t *>(ex.hidx.ptr);
     float *hs = static_cast<float *>(ex.hsim.ptr);
     const uint64_t *hl = static_cast<const uint64_t *>(ex.hlen.ptr);
-    // An exact request with k below the batch's k keeps the first k results,
-    // unless rows tie across its k-th score: the reference heap's choice
-    // among them depends on k, so such a query is searched again alone.
-    if (mode == 0) {
+    // A request with k below the batch's k keeps the first k results, unless
+    // rows tie across its k-th score: the reference heap's choice among them
+    // depends on k, so such a query is searched again alone.
+    {
         bool again = false;
         off = 0;
         for (auto &sg : batch) {
@@ -323,8 +323,14 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
                     __builtin_bit_cast(uint32_t, hs[src + rk - 1]) !=
                         __builtin_bit_cast(uint32_t, hs[src + rk]))
                     continue;
-                index_->search_device(*ex.ws, dq + (off + i) * pdims_, 1, rk, didx + src,
-                                      dsim + src, ex.stream);
+                if (mode == 0)
+                    index_->search_device(*ex.ws, dq + (off + i) * pdims_, 1, rk, didx + src,
+                                          dsim + src, ex.stream);
+                else
+                    index_->search_ivf_device(*ex.ws, dq + (off + i) * pdims_, 1, rk, nprobe,
+                                              didx + src, dsim + src,
+                                              static_cast<uint64_t *>(ex.dlen.ptr) + off + i,
+                                              ex.stream);
                 NNDEX_HIP_CHECK(hipMemcpyAsync(hi + src, didx + src, rk * 8,
                                                hipMemcpyDeviceToHost, ex.stream));
                 NNDEX_HIP_CHECK(hipMemcpyAsync(hs + src, dsim + src, rk * 4,
@@ -333,6 +339,9 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
             }
             off += sg.n;
         }
+        if (again && mode == 1)
+            NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hlen.ptr, ex.dlen.ptr, (size_t)nq * 8,
+                                           hipMemcpyDeviceToHost, ex.stream));
         if (again) NNDEX_HIP_CHECK(hipStreamSynchronize(ex.stream));
     }
     // scatter: each request keeps the first k of its rows' results

@@ -92,6 +92,42 @@ def test_ivf_search_equals_oracle(orc, rows, dims, k, nprobe):
         assert np.array_equal(bsim[i, : lens[i]].view(np.uint32), rsim.view(np.uint32))
 
 
+def test_ivf_ties_follow_the_reference_heap(orc):
+    """Duplicated rows: the reference pushes probed rows list by list (probe
+    order) into one heap that evicts the lowest original row among equal
+    minima, so among rows tied at the k-th score it keeps a set that neither
+    score-then-row nor arrival order gives.  Single, batch and device searches
+    must equal the oracle's restatement bit for bit."""
+    _case("ivf_ties")
+    rng = np.random.default_rng(6)
+    pool = rng.normal(size=(200, 24)).astype(np.float32)  # ~30 copies each, mixed lists
+    raw = pool[rng.integers(0, 200, size=6000)]
+    m = orc.normalize_rows(raw, 6000, 24)
+    qs = orc.normalize_rows(pool[rng.integers(0, 200, size=24)] +
+                            0.5 * rng.normal(size=(24, 24)).astype(np.float32), 24, 24)
+    b = BackendIndex(m, approx=True)
+    ref = orc.Ivf(m)
+    shifted = 0
+    for k, nprobe in ((5, 2), (30, 3), (60, 4)):
+        b.set_ivf(nprobe=nprobe)
+        bidx, bsim, lens = b.search_batch(qs, k, approx=True)
+        for i, q in enumerate(qs):
+            ridx, rsim = ref.search_gpu(q, k, nprobe)
+            idx, sim = b.search(q, k, approx=True)
+            np.testing.assert_array_equal(idx, ridx)
+            assert np.array_equal(sim.view(np.uint32), rsim.view(np.uint32))
+            assert int(lens[i]) == len(ridx)
+            np.testing.assert_array_equal(bidx[i, : lens[i]], ridx)
+            # score-then-row order over the probed rows, for comparison
+            lists = ref.probe(q, nprobe)
+            off, orig = ref.offsets(), ref.original_indices()
+            rows = [int(orig[r]) for c in lists for r in range(int(off[c]), int(off[c + 1]))]
+            sc = {r: orc.dot_product(m[r], q) for r in rows}
+            canon = sorted(rows, key=lambda r: (-sc[r], r))[:k]
+            shifted += canon != ridx.tolist()
+    assert shifted > 0  # the case exercises the heap's own choice
+
+
 def test_ivf_device_batch_and_nprobe(orc):
     """The device entry point with explicit nprobe over a batch above 8 queries
     (centroid probe on the certified MFMA path); nprobe = nlist equals the
