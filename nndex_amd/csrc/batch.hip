@@ -750,11 +750,11 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
         uint64_t key = 0;
         if (i < nres) {
             const uint32_t row = key_row(slots[0][i]);
-            const float4 *xb = a.x + (size_t)(row / 64) * a.d4 * 64 + row % 64;
+            const float4 *xb = a.x + (size_t)row * a.d4;
             const float4 *qb = a.q + (size_t)q * a.d4;
             float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
             for (uint32_t cc = 0; cc < a.d4; ++cc) {
-                float4 xv = xb[(size_t)cc * 64], qv = qb[cc];
+                float4 xv = xb[cc], qv = qb[cc];
                 l0 = __builtin_fmaf(xv.x, qv.x, l0);
                 l1 = __builtin_fmaf(xv.y, qv.y, l1);
                 l2 = __builtin_fmaf(xv.z, qv.z, l2);
@@ -819,10 +819,10 @@ __global__ __launch_bounds__(256) void rerank_full_kernel(RerankArgs a) {
         uint64_t key = 0;
         if (i < total) {
             const uint32_t row = key_row(c[i]);
-            const float4 *xb = a.x + (size_t)(row / 64) * a.d4 * 64 + row % 64;
+            const float4 *xb = a.x + (size_t)row * a.d4;
             float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
             for (uint32_t cc = 0; cc < a.d4; ++cc) {
-                float4 xv = xb[(size_t)cc * 64], qv = qb[cc];
+                float4 xv = xb[cc], qv = qb[cc];
                 l0 = __builtin_fmaf(xv.x, qv.x, l0);
                 l1 = __builtin_fmaf(xv.y, qv.y, l1);
                 l2 = __builtin_fmaf(xv.z, qv.z, l2);
@@ -855,6 +855,35 @@ __global__ __launch_bounds__(256) void rerank_full_kernel(RerankArgs a) {
         a.out_sim[(size_t)q * a.k + i] = key_sim(slots[0][i]);
     }
     if (lane == 0) a.flags[q] = 0;
+}
+
+// Tiled -> row-major, one block per 64-row tile: 16 chunks of the tile's 64
+// rows staged in LDS (1 KiB coalesced reads), then written row by row (16
+// threads per 256-byte row segment).
+__global__ __launch_bounds__(256) void tiled_to_rows_kernel(const float4 *__restrict__ x,
+                                                            uint32_t rows, uint32_t d4,
+                                                            float4 *__restrict__ y) {
+    __shared__ float4 t[16][64];
+    const uint32_t tile = blockIdx.x;
+    const float4 *src = x + (size_t)tile * d4 * 64;
+    for (uint32_t c0 = 0; c0 < d4; c0 += 16) {
+        const uint32_t nc = d4 - c0 < 16 ? d4 - c0 : 16;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nc * 64; i += 256)
+            t[i / 64][i % 64] = src[(size_t)(c0 + i / 64) * 64 + i % 64];
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nc * 64; i += 256) {
+            const uint32_t r = i / nc, c = i % nc, row = tile * 64 + r;
+            if (row < rows) y[(size_t)row * d4 + c0 + c] = t[c][r];
+        }
+    }
+}
+
+void launch_tiled_to_rows(const float4 *x, uint32_t rows, uint32_t d4, float4 *y, hipStream_t s) {
+    const uint32_t tiles = (rows + 63) / 64;
+    if (tiles == 0) return;
+    hipLaunchKernelGGL(tiled_to_rows_kernel, dim3(tiles), dim3(256), 0, s, x, rows, d4, y);
+    NNDEX_HIP_CHECK(hipGetLastError());
 }
 
 // Strided row sample of the tiled matrix: sample row i = row i * stride.

@@ -37,7 +37,7 @@ struct MfmaScanArgs {
 };
 
 struct RerankArgs {
-    const float4 *x;           // tiled FP32 master copy
+    const float4 *x;           // row-major FP32 copy of the master rows (d4 float4 per row)
     uint32_t d4;
     const float4 *q;           // FP32 queries, row-major, d4 float4 each
     const uint64_t *cand;
@@ -73,6 +73,9 @@ void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint3
                              hipStream_t s);
 void launch_rerank(const RerankArgs &a, uint32_t nq, hipStream_t s);
 void launch_rerank_full(const RerankArgs &a, uint32_t nq, hipStream_t s);
+// Tiled FP32 rows -> row-major (d4 float4 per row): the reranks read whole
+// candidate rows, which the tiled layout spreads over d4 separate 1 KiB lines.
+void launch_tiled_to_rows(const float4 *x, uint32_t rows, uint32_t d4, float4 *y, hipStream_t s);
 void launch_gather_rows(const float4 *x, uint32_t d4, uint32_t stride, uint32_t rows_s,
                         float4 *dst, hipStream_t s);
 void launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);

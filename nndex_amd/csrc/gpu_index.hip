@@ -168,6 +168,7 @@ GpuIndex::~GpuIndex() {
     }
     if (x_) (void)hipFree(x_);
     if (xbf16_) (void)hipFree(xbf16_);
+    if (xrow_) (void)hipFree(xrow_);
     if (xmax_sq_) (void)hipFree(xmax_sq_);
     if (xerr_sq_) (void)hipFree(xerr_sq_);
     for (void *p : {(void *)xfp8_, (void *)xscale_, (void *)xerr8_sq_})
@@ -224,6 +225,15 @@ void GpuIndex::ensure_bf16(hipStream_t s) {
     NNDEX_HIP_CHECK(hipMemsetAsync(xerr_sq_, 0, 2 * sizeof(unsigned long long), s));
     launch_bf16_error(x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(d4_), xerr_sq_ + 1,
                       s);
+    // complete before the lock is released: another stream may use it next
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void GpuIndex::ensure_rowmajor(hipStream_t s) {
+    std::lock_guard<std::mutex> l(lazy_mu_);
+    if (xrow_) return;
+    NNDEX_HIP_CHECK(hipMalloc(&xrow_, rows_ * d4_ * sizeof(float4)));
+    launch_tiled_to_rows(x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(d4_), xrow_, s);
     // complete before the lock is released: another stream may use it next
     NNDEX_HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -301,6 +311,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
     kp = plan.kp;
     if (bf16) ensure_bf16(s);
     if (fp8) ensure_fp8(s);
+    ensure_rowmajor(s);
     const RowSample &smp = sample_for(plan.stride, p, s);
     const size_t capq = plan.capq;
     const size_t group = plan.group;
@@ -350,7 +361,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         launch_mfma_scan(a, elem, grid_for(a.rows_padded), s);
         // stages 3-4: exact rerank and certificate
         RerankArgs r{};
-        r.x = x_;
+        r.x = xrow_;
         r.d4 = static_cast<uint32_t>(d4_);
         r.q = reinterpret_cast<const float4 *>(qf);
         r.cand = ws.cand;
@@ -455,6 +466,7 @@ void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hi
         // capacity are not monotone in k
         const bool bf16 = p == Precision::BF16;
         if (bf16) ensure_bf16(s);
+        ensure_rowmajor(s);
         size_t rows_s = 0, capq = 0, group = 0;
         for (size_t kk = 1; kk <= k && kk <= 400 && kk <= rows_; ++kk) {
             const BatchPlan plan = plan_batch(kk, p, 0);

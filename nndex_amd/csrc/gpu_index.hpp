@@ -239,6 +239,8 @@ class GpuIndex {
     const RowSample &sample_for(uint32_t stride, Precision p, hipStream_t s);
     void ensure_bf16(hipStream_t s);
     void ensure_fp8(hipStream_t s);
+    void ensure_rowmajor(hipStream_t s);
+    float4 *xrow_ = nullptr;                 // row-major FP32 copy (the reranks' reads)
     uint4 *xfp8_ = nullptr;                  // FP8-E4M3 copy, tiled units of 16
     float *xscale_ = nullptr;                // its per-row scales [rows_padded]
     unsigned long long *xerr8_sq_ = nullptr; // largest row rounding-error norm^2
