@@ -614,22 +614,33 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 if constexpr (E::SCALED) qs = qi < a.nq ? a.q_scale[qi] : 0.f;
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
+                    const uint32_t row0 = bt * block_rows + wid * RW + m * 16 + 4 * g;
+                    float sc[4];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
-                        float sc = E::total(acc[m][t], v);
-                        if constexpr (E::SCALED) sc = sc * (qs * rs[m][v]);
-                        if constexpr (AM) {
-                            const uint64_t key = qi < a.nq ? make_key(sc, qi) : 0;
+                        sc[v] = E::total(acc[m][t], v);
+                        if constexpr (E::SCALED) sc[v] = sc[v] * (qs * rs[m][v]);
+                    }
+                    if constexpr (AM) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const uint64_t key = qi < a.nq ? make_key(sc[v], qi) : 0;
                             if (key > best[m][v]) {
                                 second[m][v] = best[m][v];
                                 best[m][v] = key;
                             } else if (key > second[m][v]) {
                                 second[m][v] = key;
                             }
-                        } else if (qi < a.nq && row < a.rows) {
-                            a.out[(size_t)qi * a.ldo + row] = sc;
                         }
+                    } else if (qi < a.nq) {
+                        // the lane's four rows are consecutive: one 16-byte
+                        // store (ldo is a multiple of 4)
+                        float *dst = a.out + (size_t)qi * a.ldo + row0;
+                        if (row0 + 3 < a.rows)
+                            *reinterpret_cast<float4 *>(dst) = make_float4(sc[0], sc[1], sc[2], sc[3]);
+                        else
+                            for (int v = 0; v < 4; ++v)
+                                if (row0 + v < a.rows) dst[v] = sc[v];
                     }
                 }
             }
@@ -686,9 +697,20 @@ __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__re
     WaveTopK tk;
     tk.init(slots[wid], CW, kp, nullptr);
     const float *s = scores + (size_t)q * ld;
-    for (uint32_t i0 = wid * 64; i0 < n; i0 += 256) {
-        uint32_t i = i0 + lane;
-        tk.offer(i < n ? make_key(s[i], i) : 0, i < n);
+    // independent batches of 8 loads per lane before they are offered
+    constexpr int KB = 8;
+    for (uint32_t i0 = wid * 64 * KB; i0 < n; i0 += 256 * KB) {
+        float v[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const uint32_t i = i0 + u * 64 + lane;
+            v[u] = i < n ? s[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const uint32_t i = i0 + u * 64 + lane;
+            tk.offer(i < n ? make_key(v[u], i) : 0, i < n);
+        }
     }
     tk.compact();
     if (lane == 0) cnt[wid] = tk.count;
@@ -726,9 +748,16 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
 
     WaveTopK tk;
     tk.init(slots[wid], CW, kp1, nullptr);
-    for (uint32_t i0 = wid * 64; i0 < n; i0 += 256) {
-        uint32_t i = i0 + lane;
-        tk.offer(i < n ? c[i] : 0, i < n);
+    constexpr int KB = 8;  // independent batches of 8 loads per lane
+    for (uint32_t i0 = wid * 64 * KB; i0 < n; i0 += 256 * KB) {
+        uint64_t v[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const uint32_t i = i0 + u * 64 + lane;
+            v[u] = i < n ? c[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < KB; ++u) tk.offer(v[u], i0 + u * 64 + lane < n);
     }
     tk.compact();
     if (lane == 0) cnt[wid] = tk.count;

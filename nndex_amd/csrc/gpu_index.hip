@@ -60,6 +60,7 @@ static void regrow(T *&p, size_t n) {
 }
 
 void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq) {
+    rows_s = sample_ld(rows_s);
     if (nq > b_queries || nq * rows_s > b_scores || nq * capq > b_cand) ++generation;
     if (nq > b_queries) {
         regrow(qlow, nq * pdims / 8);
@@ -337,7 +338,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.nq = static_cast<uint32_t>(n);
         a.mode = 1;
         a.out = ws.scores;
-        a.ldo = smp.rows;
+        a.ldo = static_cast<uint32_t>(sample_ld(smp.rows));
         const uint32_t br = mfma_block_rows(elem, dch);
         auto grid_for = [&](uint32_t rp) {
             uint32_t blocks = (rp + br - 1) / br;
@@ -345,7 +346,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
             return blocks < cap ? blocks : cap;
         };
         launch_mfma_scan(a, elem, grid_for(a.rows_padded), s);
-        launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, smp.rows,
+        launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, a.ldo,
                                 static_cast<uint32_t>(kp), ws.delta, ws.thresh, s);
         // stage 2: filter scan over every row
         launch_zero_u32(ws.cand_count, n, s);
@@ -433,7 +434,7 @@ BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
 
 size_t BatchPlan::workspace_bytes(size_t nq, size_t pdims) const {
     const size_t n = nq < group ? nq : group;
-    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 4 + 1) + n * rows_s * 4 + n * capq * 8;
+    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 4 + 1) + n * sample_ld(rows_s) * 4 + n * capq * 8;
 }
 
 bool GpuIndex::exact_batch_precision(Precision &p) const {

@@ -29,6 +29,10 @@ enum class Precision : int { F32 = 0, BF16 = 1, FP8 = 2 };
 
 // Scratch of the fused scan: block lists, thresholds and the ticket; plus the
 // batched path's buffers, grown on demand.
+// Leading dimension of the sample score matrix: rows rounded up to 4, so a
+// lane's four consecutive rows are one aligned 16-byte store.
+inline size_t sample_ld(size_t rows_s) { return (rows_s + 3) & ~static_cast<size_t>(3); }
+
 struct Workspace {
     int device = 0;
     uint64_t generation = 0;  // bumped whenever a buffer is reallocated
