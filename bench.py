@@ -467,8 +467,8 @@ def main():
         check(L.nndex_gpu_index_set_launch(h, args.tiles_per_wave, args.scan_flags))
         tune = f"fixed flags={args.scan_flags} tiles_per_wave={args.tiles_per_wave}"
     elif not args.no_autotune:
-        buf = ctypes.create_string_buffer(4096)
-        check(L.nndex_gpu_index_autotune(h, 20, buf, 4096))
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(L.nndex_gpu_index_autotune(h, 20, buf, 1 << 16))
         tune = buf.value.decode()
         log("autotune:\n" + tune.strip())
     ws = ctypes.c_void_p()
@@ -562,6 +562,14 @@ def main():
 
     # ---- batches (exact FP32) ----
     batch = {}
+    batch_tune = None
+    if not args.no_batch and not args.no_autotune:
+        # the batch autotuner: tile shapes of the MFMA scan per precision
+        buf = ctypes.create_string_buffer(4096)
+        check(L.nndex_gpu_index_autotune_batch(h, 3, buf, 4096))
+        tuned = buf.value.decode().strip().splitlines()
+        log("batch autotune:\n" + "\n".join(tuned))
+        batch_tune = [ln for ln in tuned if ln.startswith("batch ")]
     if not args.no_batch:
         for B in (16, 256, 4096):
             reps = max(2, min(50, int(2000 // B)))
@@ -670,6 +678,7 @@ def main():
         "host_api_qps": round(host_qps, 1),
         "copy_kernel_gbs": round(copy_gbs, 1),
         "autotune": tune.strip().splitlines()[-1] if tune else None,
+        "batch_autotune": batch_tune,
         "batch": batch,
         "precision": precision,
         "ivf": ivf,

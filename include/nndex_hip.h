@@ -199,6 +199,18 @@ void nndex_graph_free(nndex_graph *graph);
  * re-created. */
 nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char *report,
                                       size_t report_len);
+/* Batch autotuner (no reference counterpart): times the batched MFMA scan's
+ * tile-shape variants (rows per wavefront, queries per LDS chunk) for each
+ * scoring precision the index supports, on a 2048-query batch of its own rows,
+ * and keeps the fastest per precision; writes a text report.  Results do not
+ * depend on the variant.  HIP graphs captured before keep their shapes. */
+nndex_status nndex_gpu_index_autotune_batch(nndex_gpu_index *index, int32_t reps, char *report,
+                                            size_t report_len);
+/* Force a batched scan tile-shape variant (0 = default, 1 = the alternative;
+ * a row width without an alternative runs the default) for one precision
+ * (NNDEX_PRECISION_*). */
+nndex_status nndex_gpu_index_set_batch_tiles(nndex_gpu_index *index, int32_t precision,
+                                             int32_t variant);
 /* Set the fused scan's launch shape: 64-row tiles per wavefront (0 =
  * default) and flags (bit 0: non-temporal loads, bit 1: contiguous tile runs
  * per wavefront instead of interleaved tiles, bits 2-3: single-query load

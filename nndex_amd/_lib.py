@@ -169,6 +169,8 @@ SIGNATURES = {
     "nndex_gpu_index_search_batch_device": (_ST, [_P, _P, _P, _SZ, _SZ, _I32, _SZ, _P, _P, _P,
                                                   _P]),
     "nndex_gpu_index_autotune": (_ST, [_P, _I32, ctypes.c_char_p, _SZ]),
+    "nndex_gpu_index_autotune_batch": (_ST, [_P, _I32, ctypes.c_char_p, _SZ]),
+    "nndex_gpu_index_set_batch_tiles": (_ST, [_P, _I32, _I32]),
     "nndex_gpu_index_set_launch": (_ST, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     "nndex_gpu_index_set_ivf_params": (_ST, [_P, _SZ, _SZ, _SZ]),
     "nndex_gpu_index_build_ivf": (_ST, [_P, ctypes.POINTER(IvfInfo)]),

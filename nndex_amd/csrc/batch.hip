@@ -1053,20 +1053,27 @@ static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
 
 // Tile shapes per step count (a row of `dch` 16-byte units runs the smallest
 // step count with 4 * steps >= dch; extra units are zero): the 16-query MFMA
-// tiles per chunk (nt) and rows per wave (rw) of each element type, for the
-// candidate and score scans and, separately, for the row argmax (whose
-// epilogue keeps two keys per row).  Chosen by timing at 1M x 128-1024 with
-// 2048 queries; every shape fits 256 VGPRs.
+// tiles per chunk (nt) and rows per wave (rw) of each element type for the
+// candidate and score scans, with an alternative shape for the autotuner
+// (variant 1; equal to the default where no other shape fits 256 VGPRs
+// without heavy spills), and the row argmax's own shapes (its epilogue keeps
+// two keys per row).  Defaults chosen by timing at 1M x 128-1024 with 2048
+// queries.
 struct MfmaTiles {
     uint32_t steps;
-    int f32_nt, bf_rw, bf_nt, f8_rw, f8_nt;  // mode 0 / 1
+    int f32_nt, bf_rw, bf_nt, f8_rw, f8_nt;  // mode 0 / 1, variant 0
+    int f32_nt1, bf_rw1, bf_nt1, f8_rw1, f8_nt1;  // variant 1
     int am_f32_nt, am_bf_rw, am_bf_nt;       // mode 2
 };
 static constexpr MfmaTiles kMfmaTiles[] = {
-    {2, 4, 32, 4, 32, 4, 4, 32, 4},   {4, 4, 32, 4, 32, 2, 4, 32, 4},
-    {6, 2, 32, 2, 32, 2, 2, 32, 2},   {8, 2, 32, 2, 16, 2, 2, 32, 2},
-    {12, 2, 32, 2, 32, 2, 2, 16, 4},  {16, 2, 32, 1, 32, 1, 2, 16, 2},
-    {24, 2, 16, 2, 16, 1, 1, 16, 1},  {32, 1, 16, 1, 16, 1, 1, 16, 1},
+    {2, 4, 32, 4, 32, 4, 2, 32, 2, 32, 2, 4, 32, 4},
+    {4, 4, 32, 4, 32, 2, 2, 32, 2, 16, 2, 4, 32, 4},
+    {6, 2, 32, 2, 32, 2, 4, 32, 4, 16, 2, 2, 32, 2},
+    {8, 2, 32, 2, 16, 2, 4, 32, 4, 32, 2, 2, 32, 2},
+    {12, 2, 32, 2, 32, 2, 1, 16, 4, 16, 2, 2, 16, 4},
+    {16, 2, 32, 1, 32, 1, 1, 16, 2, 16, 2, 2, 16, 2},
+    {24, 2, 16, 2, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},
+    {32, 1, 16, 1, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},
 };
 constexpr int kMfmaTileCount = sizeof(kMfmaTiles) / sizeof(kMfmaTiles[0]);
 
@@ -1076,17 +1083,27 @@ static int mfma_tile_index(uint32_t dch) {
     return -1;
 }
 
-uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax) {
+uint32_t mfma_variants(int elem, uint32_t dch) {
+    const int i = mfma_tile_index(dch);
+    if (i < 0) return 1;
+    const MfmaTiles &t = kMfmaTiles[i];
+    if (elem == kElemFP8) return t.f8_rw1 != t.f8_rw || t.f8_nt1 != t.f8_nt ? 2 : 1;
+    if (elem == kElemBF16) return t.bf_rw1 != t.bf_rw || t.bf_nt1 != t.bf_nt ? 2 : 1;
+    return t.f32_nt1 != t.f32_nt ? 2 : 1;
+}
+
+uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax, int variant) {
     const int i = mfma_tile_index(dch);
     if (i < 0) return 8 * 16;
     const MfmaTiles &t = kMfmaTiles[i];
-    if (elem == kElemFP8) return 8 * t.f8_rw;
-    if (elem == kElemBF16) return 8 * (argmax ? t.am_bf_rw : t.bf_rw);
+    if (elem == kElemFP8) return 8 * (variant ? t.f8_rw1 : t.f8_rw);
+    if (elem == kElemBF16) return 8 * (argmax ? t.am_bf_rw : (variant ? t.bf_rw1 : t.bf_rw));
     return 8 * 16;
 }
 
 template <int I>
-static void launch_mfma_case(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s) {
+static void launch_mfma_case(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s,
+                             int variant) {
     constexpr MfmaTiles t = kMfmaTiles[I];
     constexpr int ST = (int)t.steps;
     if (a.mode == 2) {
@@ -1095,21 +1112,31 @@ static void launch_mfma_case(const MfmaScanArgs &a, int elem, uint32_t grid, hip
             return launch_mfma_t<BF16E, 8, t.am_bf_rw, ST, t.am_bf_nt, true>(a, grid, s);
         throw Error(NNDEX_ERR_UNSUPPORTED, "no FP8 row argmax scan");
     }
-    if (elem == kElemFP8) return launch_mfma_t<FP8E, 8, t.f8_rw, ST, t.f8_nt>(a, grid, s);
-    if (elem == kElemBF16) return launch_mfma_t<BF16E, 8, t.bf_rw, ST, t.bf_nt>(a, grid, s);
+    if (elem == kElemFP8) {
+        if (variant) return launch_mfma_t<FP8E, 8, t.f8_rw1, ST, t.f8_nt1>(a, grid, s);
+        return launch_mfma_t<FP8E, 8, t.f8_rw, ST, t.f8_nt>(a, grid, s);
+    }
+    if (elem == kElemBF16) {
+        if (variant) return launch_mfma_t<BF16E, 8, t.bf_rw1, ST, t.bf_nt1>(a, grid, s);
+        return launch_mfma_t<BF16E, 8, t.bf_rw, ST, t.bf_nt>(a, grid, s);
+    }
+    if (variant) return launch_mfma_t<F32E, 8, 16, ST, t.f32_nt1>(a, grid, s);
     return launch_mfma_t<F32E, 8, 16, ST, t.f32_nt>(a, grid, s);
 }
 
-void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s) {
+void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s,
+                      int variant) {
+    if (variant < 0 || variant > 1 || (uint32_t)variant >= mfma_variants(elem, a.dch))
+        variant = 0;
     switch (mfma_tile_index(a.dch)) {
-    case 0: return launch_mfma_case<0>(a, elem, grid, s);
-    case 1: return launch_mfma_case<1>(a, elem, grid, s);
-    case 2: return launch_mfma_case<2>(a, elem, grid, s);
-    case 3: return launch_mfma_case<3>(a, elem, grid, s);
-    case 4: return launch_mfma_case<4>(a, elem, grid, s);
-    case 5: return launch_mfma_case<5>(a, elem, grid, s);
-    case 6: return launch_mfma_case<6>(a, elem, grid, s);
-    case 7: return launch_mfma_case<7>(a, elem, grid, s);
+    case 0: return launch_mfma_case<0>(a, elem, grid, s, variant);
+    case 1: return launch_mfma_case<1>(a, elem, grid, s, variant);
+    case 2: return launch_mfma_case<2>(a, elem, grid, s, variant);
+    case 3: return launch_mfma_case<3>(a, elem, grid, s, variant);
+    case 4: return launch_mfma_case<4>(a, elem, grid, s, variant);
+    case 5: return launch_mfma_case<5>(a, elem, grid, s, variant);
+    case 6: return launch_mfma_case<6>(a, elem, grid, s, variant);
+    case 7: return launch_mfma_case<7>(a, elem, grid, s, variant);
     default:
         throw Error(NNDEX_ERR_UNSUPPORTED, "padded dims above 512 (f32) / 1024 (bf16) / 2048 "
                                            "(fp8) are not supported by the MFMA scan");

@@ -65,9 +65,13 @@ void launch_prep_queries(const float *q, uint32_t nq, uint32_t pdims, int elem, 
                          const unsigned long long *xerr_sq, float *delta, hipStream_t s);
 void launch_bf16_error(const float4 *x, uint32_t rows, uint32_t d4, unsigned long long *out,
                        hipStream_t s);
-// Rows per block of the MFMA scan (the argmax variant has its own tiles).
-uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax = false);
-void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s);
+// Tile-shape variants of the candidate / score scan for a row width (1 or 2;
+// the autotuner times them), rows per block of a variant (the argmax scan
+// has its own tiles), and the launch.
+uint32_t mfma_variants(int elem, uint32_t dch);
+uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax = false, int variant = 0);
+void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s,
+                      int variant = 0);
 void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
                              uint32_t kp, const float *delta, unsigned long long *thresh,
                              hipStream_t s);

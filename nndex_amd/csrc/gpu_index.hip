@@ -339,13 +339,14 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.mode = 1;
         a.out = ws.scores;
         a.ldo = static_cast<uint32_t>(sample_ld(smp.rows));
-        const uint32_t br = mfma_block_rows(elem, dch);
+        const int variant = mfma_variant_[elem].load(std::memory_order_relaxed);
+        const uint32_t br = mfma_block_rows(elem, dch, false, variant);
         auto grid_for = [&](uint32_t rp) {
             uint32_t blocks = (rp + br - 1) / br;
             uint32_t cap = ncu_;  // one 8-wave block per CU (register-bound)
             return blocks < cap ? blocks : cap;
         };
-        launch_mfma_scan(a, elem, grid_for(a.rows_padded), s);
+        launch_mfma_scan(a, elem, grid_for(a.rows_padded), s, variant);
         launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, a.ldo,
                                 static_cast<uint32_t>(kp), ws.delta, ws.thresh, s);
         // stage 2: filter scan over every row
@@ -359,7 +360,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.cand = ws.cand;
         a.cand_count = ws.cand_count;
         a.capq = static_cast<uint32_t>(capq);
-        launch_mfma_scan(a, elem, grid_for(a.rows_padded), s);
+        launch_mfma_scan(a, elem, grid_for(a.rows_padded), s, variant);
         // stages 3-4: exact rerank and certificate
         RerankArgs r{};
         r.x = xrow_;
@@ -665,6 +666,64 @@ std::string GpuIndex::autotune(int reps) {
     ws_.reset();  // grids changed; the host path re-creates its workspace
     report += "chosen flags=" + std::to_string(best_f) + " tiles_per_wave=" +
               std::to_string(best_m) + "\n";
+    return report;
+}
+
+std::string GpuIndex::autotune_batch(int reps) {
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    std::lock_guard<std::mutex> lock(mu_);
+    ensure_rowmajor(stream_);
+    // the index's first rows as queries (row-major FP32, already normalised)
+    const size_t nq = rows_ < 2048 ? rows_ : 2048, k = rows_ < 10 ? rows_ : 10;
+    const float *dq = reinterpret_cast<const float *>(xrow_);
+    DeviceBuffer di, ds, df;
+    di.reserve(nq * k * sizeof(uint64_t));
+    ds.reserve(nq * k * sizeof(float));
+    df.reserve(nq);
+    auto ws = make_workspace(nq, k);
+    hipEvent_t e0, e1;
+    NNDEX_HIP_CHECK(hipEventCreate(&e0));
+    NNDEX_HIP_CHECK(hipEventCreate(&e1));
+    std::string report;
+    for (Precision p : {Precision::F32, Precision::BF16, Precision::FP8}) {
+        if (!mfma_supported(p)) continue;
+        const int elem = static_cast<int>(p);
+        const uint32_t nv = mfma_variants(elem, static_cast<uint32_t>(mfma_units(pdims_, p)));
+        float best_ms = 1e30f;
+        int best_v = 0;
+        for (uint32_t v = 0; v < nv; ++v) {
+            mfma_variant_[elem] = static_cast<int>(v);
+            auto run = [&] {
+                search_batch_mfma(*ws, dq, nq, k, p, 0, static_cast<uint64_t *>(di.ptr),
+                                  static_cast<float *>(ds.ptr), static_cast<uint8_t *>(df.ptr),
+                                  stream_);
+            };
+            run();
+            NNDEX_HIP_CHECK(hipEventRecord(e0, stream_));
+            for (int i = 0; i < reps; ++i) run();
+            NNDEX_HIP_CHECK(hipEventRecord(e1, stream_));
+            NNDEX_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            NNDEX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            static const char *names[] = {"f32", "bf16", "fp8"};
+            report += std::string("batch ") + names[elem] + " variant=" + std::to_string(v) +
+                      " rows_per_block=" +
+                      std::to_string(mfma_block_rows(elem,
+                                                     static_cast<uint32_t>(mfma_units(pdims_, p)),
+                                                     false, static_cast<int>(v))) +
+                      " ms=" + std::to_string(ms) + "\n";
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_v = static_cast<int>(v);
+            }
+        }
+        mfma_variant_[elem] = best_v;
+        report += std::string("chosen ") + std::to_string(elem) + " variant=" +
+                  std::to_string(best_v) + "\n";
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return report;
 }
 
@@ -1217,6 +1276,29 @@ nndex_status nndex_gpu_index_autotune(nndex_gpu_index *index, int32_t reps, char
             std::memcpy(report, r.data(), n);
             report[n] = 0;
         }
+    });
+}
+
+nndex_status nndex_gpu_index_autotune_batch(nndex_gpu_index *index, int32_t reps, char *report,
+                                           size_t report_len) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        std::string r = index->impl->autotune_batch(reps > 0 ? reps : 3);
+        if (report && report_len) {
+            size_t n = r.size() < report_len - 1 ? r.size() : report_len - 1;
+            std::memcpy(report, r.data(), n);
+            report[n] = 0;
+        }
+    });
+}
+
+nndex_status nndex_gpu_index_set_batch_tiles(nndex_gpu_index *index, int32_t precision,
+                                             int32_t variant) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (precision < 0 || precision > 2 || variant < 0 || variant > 1)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "precision must be 0-2 and variant 0-1");
+        index->impl->set_batch_variant(static_cast<Precision>(precision), variant);
     });
 }
 

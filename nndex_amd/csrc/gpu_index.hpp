@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <deque>
 #include <memory>
@@ -215,6 +216,12 @@ class GpuIndex {
     // Time the fused scan over candidate launch shapes and keep the fastest;
     // returns a one-line report per candidate.
     std::string autotune(int reps);
+    // Times the batched MFMA scan's tile-shape variants per precision on
+    // this index (its first rows as a 2048-query batch) and keeps the
+    // fastest; results do not depend on the variant.
+    std::string autotune_batch(int reps);
+    void set_batch_variant(Precision p, int v) { mfma_variant_[static_cast<int>(p)] = v; }
+    int batch_variant(Precision p) const { return mfma_variant_[static_cast<int>(p)]; }
 
   private:
     GpuIndex() = default;
@@ -231,6 +238,7 @@ class GpuIndex {
     int device_ = 0;
     uint32_t ncu_ = 0;
     uint32_t scan_tpw_ = 0;
+    std::atomic<int> mfma_variant_[3] = {{0}, {0}, {0}};  // per Precision
     uint32_t scan_flags_ = kScanNonTemporal;
     size_t rows_ = 0, pdims_ = 0, d4_ = 0;
     bool approx_ = false, enable_cache_ = true;

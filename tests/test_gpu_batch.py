@@ -196,3 +196,32 @@ def test_batch_duplicate_rows_ties_at_threshold(orc, precision):
     for i in range(48):
         oidx, osim = orc.gpu_exact_search(om, oq[i], 10)
         assert bidx[i].tolist() == oidx.tolist() and bsim[i].tolist() == osim.tolist(), i
+
+
+@pytest.mark.parametrize("dims", [128, 384])
+def test_batch_tile_variants_agree_and_autotune(dims):
+    """The batch autotuner's tile-shape variants (rows per wavefront, queries
+    per chunk) give identical results and flags for every precision; the
+    autotuner reports its timings and a choice per precision."""
+    _case("batch_variants", dims=dims)
+    base, queries = synth.mixture_workload(rows=60_000, dims=dims, n_queries=300)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 300, 64)
+    L = _lib.lib()
+    for prec in (0, 1, 2):
+        out = []
+        for v in (0, 1):
+            check(L.nndex_gpu_index_set_batch_tiles(b.h, prec, v))
+            out.append(batch_device(b, ds, queries, 10, prec))
+        for x, y in zip(out[0], out[1]):
+            assert np.array_equal(x, y), prec
+    buf = ctypes.create_string_buffer(4096)
+    check(L.nndex_gpu_index_autotune_batch(b.h, 1, buf, 4096))
+    report = buf.value.decode()
+    print(report, flush=True)
+    assert report.count("chosen") == 3
+    ei, es = exact_device(b, ds, queries, 10)
+    gi, gs, fl = batch_device(b, ds, queries, 10, 0)
+    for i in range(300):
+        if not fl[i]:
+            assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
