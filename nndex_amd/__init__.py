@@ -164,6 +164,23 @@ class NNdex:
     def padded_dims(self) -> int:
         return (self._dims + 7) // 8 * 8
 
+    def autotune(self, reps: int = 20, batch: bool = True) -> str:
+        """Time the launch shapes of this GPU index's kernels on its own data
+        and keep the fastest: the single-query scan (64-row tiles per
+        wavefront, load flags) and, with ``batch``, the batched MFMA scan's
+        tile shapes per scoring precision.  Returns the timing report.
+        Results do not depend on the shapes chosen."""
+        h = self.backend_handle
+        if h is None:
+            raise RuntimeError("autotune needs a GPU index")
+        buf = ctypes.create_string_buffer(1 << 16)
+        _lib.check(_lib.lib().nndex_gpu_index_autotune(h, reps, buf, 1 << 16))
+        report = buf.value.decode()
+        if batch:
+            _lib.check(_lib.lib().nndex_gpu_index_autotune_batch(h, 3, buf, 1 << 16))
+            report += buf.value.decode()
+        return report
+
     @property
     def gpu_info(self) -> dict | None:
         info = _lib.GpuDeviceInfo()

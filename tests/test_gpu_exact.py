@@ -315,3 +315,19 @@ def test_full_size_1m_x_384(orc, mixture_1m):
         rows = base[gi[i].astype(np.int64)]
         direct = [orc.dot_product(r, queries[i]) for r in rows]
         assert direct == gs[i].tolist()
+
+
+def test_frontend_autotune_keeps_results(orc):
+    """NNdex.autotune times the scan and batch tile shapes on the index's own
+    data; searches before and after return the same results."""
+    _print_case("frontend_autotune")
+    rng = np.random.default_rng(21)
+    m = rng.normal(size=(30_000, 96)).astype(np.float32)
+    qs = rng.normal(size=(40, 96)).astype(np.float32)
+    index = nndex_amd.NNdex(m, normalized=False, backend="gpu", enable_cache=False)
+    before = index.search(qs, k=10), index.search(qs[0], k=10)
+    report = index.autotune(reps=2)
+    assert "chosen flags=" in report and report.count("chosen ") >= 3
+    after = index.search(qs, k=10), index.search(qs[0], k=10)
+    for (bi, bs), (ai, as_) in zip(before, after):
+        assert np.array_equal(bi, ai) and np.array_equal(bs, as_)
