@@ -991,7 +991,7 @@ const IvfIndex &GpuIndex::build_ivf() {
     NNDEX_HIP_CHECK(hipMemcpy(&bits, xmax_sq_, sizeof(bits), hipMemcpyDeviceToHost));
     const double x_max = std::sqrt(__builtin_bit_cast(double, bits));
     double x_err = 0.0;
-    if (pdims_ > 512) {  // BF16 assignment
+    {  // BF16 copy: the k-means++ pre-test, and the assignment above 512 dims
         ensure_bf16(stream_);
         NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
         NNDEX_HIP_CHECK(hipMemcpy(&ebits, xerr_sq_ + 1, sizeof(ebits), hipMemcpyDeviceToHost));

@@ -143,22 +143,66 @@ __global__ void copy_key_row_kernel(const float4 *__restrict__ x, uint32_t d4,
 
 // ---- k-means++ seeding (kmeans_plusplus_init, approx.rs:357-406) ------------
 
+// The new centroid's values rounded to BF16 (as the BF16 row copy rounds).
+__global__ void bf16_round_row_kernel(const float4 *__restrict__ v, uint32_t d4,
+                                      float4 *__restrict__ out) {
+    for (uint32_t c = threadIdx.x; c < d4; c += blockDim.x) {
+        const float4 a = v[c];
+        auto r = [](float f) { return (float)(__bf16)f; };
+        out[c] = make_float4(r(a.x), r(a.y), r(a.z), r(a.w));
+    }
+}
+
 // min_dist[r] = min(min_dist[r], max(0, 1 - dot(row, last))); f64 block sums.
+//
+// With the BF16 row copy (xb, d8 units of 8 per row; lastb the centroid
+// rounded to BF16) a row is first scored approximately: a = x~.c~ in f32 is
+// within delta of x.c, and the reference's f32 value 1 - x.c within eps of
+// exact, so (1 - a) - (delta + eps) > m proves `d < m` false and the FP32 row
+// is not read (margin = delta + eps, from the largest row norm and BF16
+// rounding error).  Other rows take the reference's computation.
 __global__ __launch_bounds__(256) void kpp_update_kernel(const float4 *__restrict__ x,
+                                                         const uint4 *__restrict__ xb,
                                                          uint32_t rows, uint32_t d4,
                                                          const float4 *__restrict__ last,
+                                                         const float4 *__restrict__ lastb,
+                                                         double margin,
                                                          float *__restrict__ min_dist,
                                                          double *__restrict__ partial) {
     __shared__ double red[256];
     const uint32_t row = blockIdx.x * 256 + threadIdx.x;
     double md = 0.0;
     if (row < rows) {
-        float d = 1.0f - tiled_dot(x, row, d4, last);
-        if (d < 0.0f) d = 0.0f;
         float m = min_dist[row];
-        if (d < m) {
-            m = d;
-            min_dist[row] = m;
+        bool skip = false;
+        if (xb && m < 3.0e38f) {
+            const uint32_t d8 = d4 / 2;
+            const uint4 *b = xb + (size_t)(row / 64) * d8 * 64 + row % 64;
+            const_as<f32x4> *cb = as_constant(reinterpret_cast<const f32x4 *>(lastb));
+            float l[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (uint32_t c = 0; c < d8; ++c) {
+                const uint4 w = b[(size_t)c * 64];
+                const f32x4 q0 = cb[2 * c], q1 = cb[2 * c + 1];
+                const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+                const float qs[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float lo = __builtin_bit_cast(float, ws[j] << 16);
+                    const float hi = __builtin_bit_cast(float, ws[j] & 0xFFFF0000u);
+                    l[2 * j] = __builtin_fmaf(lo, qs[2 * j], l[2 * j]);
+                    l[2 * j + 1] = __builtin_fmaf(hi, qs[2 * j + 1], l[2 * j + 1]);
+                }
+            }
+            const float a = ((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + (l[6] + l[7]));
+            skip = (1.0 - (double)a) - margin > (double)m * (1.0 + 1e-12) + 1e-30;
+        }
+        if (!skip) {
+            float d = 1.0f - tiled_dot(x, row, d4, last);
+            if (d < 0.0f) d = 0.0f;
+            if (d < m) {
+                m = d;
+                min_dist[row] = m;
+            }
         }
         md = (double)m;
     }
@@ -715,6 +759,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
     const uint32_t cs_rows = 4096, cs_blocks = (rows + cs_rows - 1) / cs_rows;
     double *partial = nullptr, *colpart = nullptr;
     float *min_dist = nullptr, *mean = nullptr;
+    float4 *lastb = nullptr;
     unsigned long long *rng = nullptr, *medoid = nullptr;
     uint32_t *sorted_keys = nullptr, *rows_iota = nullptr, *counts = nullptr, *recheck = nullptr,
              *recheck_count = nullptr;
@@ -727,7 +772,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         for (void *p : {(void *)partial, (void *)colpart, (void *)min_dist, (void *)mean,
                         (void *)rng, (void *)medoid, (void *)sorted_keys, (void *)rows_iota,
                         (void *)counts, (void *)recheck, (void *)recheck_count, (void *)cbf16,
-                        (void *)cerr, (void *)tol, sort_tmp})
+                        (void *)cerr, (void *)tol, (void *)lastb, sort_tmp})
             if (p) (void)hipFree(p);
     };
     try {
@@ -744,6 +789,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         NNDEX_HIP_CHECK(hipMalloc(&recheck_count, sizeof(uint32_t)));
         NNDEX_HIP_CHECK(hipMalloc(&cerr, 2 * sizeof(unsigned long long)));
         NNDEX_HIP_CHECK(hipMalloc(&tol, sizeof(float)));
+        NNDEX_HIP_CHECK(hipMalloc(&lastb, d4 * sizeof(float4)));
         if (bf16) NNDEX_HIP_CHECK(hipMalloc(&cbf16, (size_t)nlist * d8 * sizeof(uint4)));
         uint32_t bits = 1;
         while ((1u << bits) < nlist) ++bits;
@@ -766,9 +812,23 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         NNDEX_HIP_CHECK(hipMemcpyAsync(rng, &seed, sizeof(seed), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(fill_f32_kernel, dim3(1024), dim3(256), 0, s, min_dist, (size_t)rows,
                            3.40282347e38f);
+        // margin of the BF16 pre-test: delta = gamma_{d8+3} |x~||c~| + 2 xe (xm +
+        // xe) for the approximate dot (8 FMA chains of d8 terms), eps =
+        // gamma_{d4+2} xm^2 + u (1 + xm^2) for the reference's f32 value
+        const double u = 0x1p-24, k8 = (double)d8 + 3.0, k4 = (double)d4 + 2.0;
+        const double xl = x_max + x_err;
+        const double kpp_margin = (k8 * u / (1.0 - k8 * u) * xl * xl + 2.0 * x_err * xl +
+                                   k4 * u / (1.0 - k4 * u) * x_max * x_max +
+                                   1.01 * u * (1.0 + x_max * x_max)) *
+                                  (1.0 + 1e-6);
+        const uint4 *kpp_xb = (xbf16 && d4 % 2 == 0) ? xbf16 : nullptr;
         for (uint32_t c = 1; c < nlist; ++c) {
-            hipLaunchKernelGGL(kpp_update_kernel, dim3(nb), dim3(256), 0, s, x, rows, d4,
-                               cent + (size_t)(c - 1) * d4, min_dist, partial);
+            if (kpp_xb)
+                hipLaunchKernelGGL(bf16_round_row_kernel, dim3(1), dim3(64), 0, s,
+                                   cent + (size_t)(c - 1) * d4, d4, lastb);
+            hipLaunchKernelGGL(kpp_update_kernel, dim3(nb), dim3(256), 0, s, x, kpp_xb, rows, d4,
+                               cent + (size_t)(c - 1) * d4, lastb, kpp_margin, min_dist,
+                               partial);
             hipLaunchKernelGGL(kpp_choose_kernel, dim3(1), dim3(256), 0, s, x, rows, d4, min_dist,
                                partial, nb, rng, cent, c);
         }
