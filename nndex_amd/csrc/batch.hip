@@ -512,12 +512,20 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 xr[m][s] = c < dch ? base[(size_t)c * 64] : make_uint4(0, 0, 0, 0);
             }
         }
-        uint64_t best[MT][4], second[MT][4];  // row argmax variant only
+        // row argmax variant only: the lane's best score and query, and the
+        // second-best score (queries arrive in increasing order, so `>` keeps
+        // the lowest query among equal scores; equal top scores differ by 0
+        // and go to the exact re-check)
+        float best[MT][4], second[MT][4];
+        uint32_t bestq[MT][4];
         if constexpr (AM) {
 #pragma unroll
             for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) best[m][v] = second[m][v] = 0;
+                for (int v = 0; v < 4; ++v) {
+                    best[m][v] = second[m][v] = -__builtin_inff();
+                    bestq[m][v] = 0xFFFFFFFFu;
+                }
         }
         float rs[MT][4];  // FP8 row scales of the lane's rows
 #pragma unroll
@@ -624,13 +632,11 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     if constexpr (AM) {
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const uint64_t key = qi < a.nq ? make_key(sc[v], qi) : 0;
-                            if (key > best[m][v]) {
-                                second[m][v] = best[m][v];
-                                best[m][v] = key;
-                            } else if (key > second[m][v]) {
-                                second[m][v] = key;
-                            }
+                            const float sv = qi < a.nq ? sc[v] : -__builtin_inff();
+                            const bool t = sv > best[m][v];
+                            second[m][v] = t ? best[m][v] : __builtin_fmaxf(second[m][v], sv);
+                            bestq[m][v] = t ? qi : bestq[m][v];
+                            best[m][v] = t ? sv : best[m][v];
                         }
                     } else if (qi < a.nq) {
                         // the lane's four rows are consecutive: one 16-byte
@@ -659,17 +665,21 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             for (int m = 0; m < MT; ++m)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    uint64_t b = best[m][v], c = second[m][v];
+                    float bs = best[m][v], ss = second[m][v];
+                    uint32_t bq = bestq[m][v];
                     for (int off = 1; off < 16; off <<= 1) {
-                        const uint64_t ob = __shfl_xor(b, off), oc = __shfl_xor(c, off);
-                        const uint64_t lo = ob < b ? ob : b, hc = oc > c ? oc : c;
-                        b = ob > b ? ob : b;
-                        c = lo > hc ? lo : hc;
+                        const float obs = __shfl_xor(bs, off), oss = __shfl_xor(ss, off);
+                        const uint32_t obq = __shfl_xor(bq, off);
+                        const bool take = obs > bs || (obs == bs && obq < bq);
+                        const float lo = take ? bs : obs;  // the loser's best
+                        ss = __builtin_fmaxf(lo, __builtin_fmaxf(ss, oss));
+                        bq = take ? obq : bq;
+                        bs = take ? obs : bs;
                     }
                     const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
                     if (l16 == 0 && row < a.rows) {
-                        a.assign[row] = key_row(b);
-                        if (c != 0 && !(key_sim(b) - key_sim(c) > tol))
+                        a.assign[row] = bq;
+                        if (ss > -__builtin_inff() && !(bs - ss > tol))
                             a.recheck[atomicAdd(a.recheck_count, 1u)] = row;
                     }
                 }
