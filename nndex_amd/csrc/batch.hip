@@ -354,6 +354,7 @@ constexpr uint32_t kCandQueue = 128;
 
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM>
 __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
+    static_assert(!(AM && E::SCALED), "the row argmax takes unscaled scores");
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
     constexpr uint32_t QC = 16 * NT;     // queries per chunk
     // LDS slots per unit row: unpadded, so the 16 lanes of a group read one
