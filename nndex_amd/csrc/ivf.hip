@@ -161,7 +161,40 @@ __global__ void bf16_round_row_kernel(const float4 *__restrict__ v, uint32_t d4,
 // exact, so (1 - a) - (delta + eps) > m proves `d < m` false and the FP32 row
 // is not read (margin = delta + eps, from the largest row norm and BF16
 // rounding error).  Other rows take the reference's computation.
-__global__ __launch_bounds__(256) void kpp_update_kernel(const float4 *__restrict__ x,
+constexpr uint32_t kKppRows = 256;    // rows per k-means++ update block
+constexpr uint32_t kKppBlock = 1024;  // threads of the choice kernel
+
+// Sum over a block of NT threads (one value per thread); thread 0 gets the
+// total.
+template <uint32_t NT>
+__device__ inline double kpp_block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (uint32_t i = 0; i < NT / 64; ++i) t += red[i];
+    return t;
+}
+
+// Inclusive prefix sum over the block of kKppBlock threads (wave scans, then
+// the wave totals in order).  `red` holds kKppBlock / 64 doubles; the call
+// ends with a barrier, so `red` may be reused at once.
+__device__ inline double kpp_block_scan(double v, double *red) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) red[w] = v;
+    __syncthreads();
+    double off = 0.0;
+    for (uint32_t i = 0; i < w; ++i) off += red[i];
+    __syncthreads();
+    return off + v;
+}
+
+__global__ __launch_bounds__(kKppRows) void kpp_update_kernel(const float4 *__restrict__ x,
                                                          const uint4 *__restrict__ xb,
                                                          uint32_t rows, uint32_t d4,
                                                          const float4 *__restrict__ last,
@@ -169,8 +202,8 @@ __global__ __launch_bounds__(256) void kpp_update_kernel(const float4 *__restric
                                                          double margin,
                                                          float *__restrict__ min_dist,
                                                          double *__restrict__ partial) {
-    __shared__ double red[256];
-    const uint32_t row = blockIdx.x * 256 + threadIdx.x;
+    __shared__ double red[kKppRows / 64];
+    const uint32_t row = blockIdx.x * kKppRows + threadIdx.x;
     double md = 0.0;
     if (row < rows) {
         float m = min_dist[row];
@@ -206,112 +239,105 @@ __global__ __launch_bounds__(256) void kpp_update_kernel(const float4 *__restric
         }
         md = (double)m;
     }
-    red[threadIdx.x] = md;
-    __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    const double t = kpp_block_sum<kKppRows>(md, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// One block: total, generator step, then the first row whose cumulative
-// distance reaches the target, found top-down (ranges of blocks, blocks,
-// rows) with each sequential walk over LDS.  Writes centroid c.
-__global__ __launch_bounds__(256) void kpp_choose_kernel(const float4 *__restrict__ x,
-                                                         uint32_t rows, uint32_t d4,
-                                                         const float *__restrict__ min_dist,
-                                                         const double *__restrict__ partial,
-                                                         uint32_t nblocks,
-                                                         unsigned long long *__restrict__ rng,
-                                                         float4 *__restrict__ centroids,
-                                                         uint32_t c) {
-    __shared__ double buf[256];
+// One block of kKppBlock threads: total, generator step, then the first row
+// whose cumulative distance reaches the target, found top-down (ranges of
+// update blocks, update blocks, rows) with block-wide prefix sums.  Writes
+// centroid c and, when `lastb` is set, its BF16 rounding for the next update
+// pass.
+__global__ __launch_bounds__(kKppBlock) void kpp_choose_kernel(
+    const float4 *__restrict__ x, uint32_t rows, uint32_t d4, const float *__restrict__ min_dist,
+    const double *__restrict__ partial, uint32_t nblocks, unsigned long long *__restrict__ rng,
+    float4 *__restrict__ centroids, uint32_t c, float4 *__restrict__ lastb) {
+    __shared__ double red[kKppBlock / 64];
     __shared__ double base, target;
-    __shared__ int sel, dup, found;
-    __shared__ uint32_t blk, chosen;
+    __shared__ int dup;
+    __shared__ uint32_t first, chosen;
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (nblocks + 255) / 256;
+    // this thread's range of update blocks, summed in order (loads batched)
+    const uint32_t per = (nblocks + kKppBlock - 1) / kKppBlock;
+    const uint32_t b0 = t * per < nblocks ? t * per : nblocks;
+    const uint32_t b1 = b0 + per < nblocks ? b0 + per : nblocks;
+    double own = 0.0;
     {
-        double s = 0;
-        const uint32_t b0 = t * per, b1 = (t + 1) * per < nblocks ? (t + 1) * per : nblocks;
-        for (uint32_t b = b0; b < b1; ++b) s += partial[b];
-        buf[t] = s;
+        uint32_t b = b0;
+        for (; b + 8 <= b1; b += 8) {
+            double v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = partial[b + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) own += v[j];
+        }
+        for (; b < b1; ++b) own += partial[b];
     }
-    __syncthreads();
-    if (t == 0) {
-        double total = 0;
-        for (int i = 0; i < 256; ++i) total += buf[i];
+    if (t == 0) first = 0xFFFFFFFFu;
+    double incl = kpp_block_scan(own, red);
+    if (t == kKppBlock - 1) {
+        const double total = incl;
         dup = !(total > 0.0);
-        sel = -1;
-        base = 0;
         if (!dup) {
             const unsigned long long st = *rng * 6364136223846793005ull + 1ull;
             *rng = st;
             target = ((double)st / (double)0xFFFFFFFFFFFFFFFFull) * total;
-            double cum = 0;
-            for (int i = 0; i < 256; ++i) {
-                if (cum + buf[i] >= target) {
-                    sel = i;
-                    break;
-                }
-                cum += buf[i];
-            }
-            base = cum;
         }
-        chosen = rows - 1;
+        chosen = rows - 1;  // rounding fallback
     }
     __syncthreads();
     float4 *dst = centroids + (size_t)c * d4;
     if (dup) {
         for (uint32_t k = t; k < d4; k += blockDim.x) dst[k] = centroids[(size_t)(c - 1) * d4 + k];
-        return;
+        return;  // lastb already holds that centroid's rounding
     }
-    if (sel >= 0) {
-        // the block holding the crossing, 256 block sums at a time
-        const uint32_t b0 = sel * per, b1 = b0 + per < nblocks ? b0 + per : nblocks;
-        if (t == 0) blk = b1 - 1;  // rounding fallback: the range's last block
-        for (uint32_t j0 = b0; j0 < b1; j0 += 256) {
-            const uint32_t n = b1 - j0 < 256 ? b1 - j0 : 256;
+    if (incl >= target) atomicMin(&first, t);
+    __syncthreads();
+    const uint32_t sel = first;
+    // the last thread's scan value is the total, so some thread is found
+    const uint32_t r0b = sel < kKppBlock ? sel * per : nblocks;
+    if (r0b < nblocks) {
+        // the update block holding the crossing, kKppBlock block sums at a time
+        if (t == sel) base = incl - own;
+        const uint32_t r1b = r0b + per < nblocks ? r0b + per : nblocks;
+        uint32_t blk = r1b - 1;  // rounding fallback: the range's last block
+        for (uint32_t j0 = r0b; j0 < r1b; j0 += kKppBlock) {
             __syncthreads();
-            if (t < n) buf[t] = partial[j0 + t];
+            const double start = base;
+            if (t == 0) first = 0xFFFFFFFFu;
+            const double v = j0 + t < r1b ? partial[j0 + t] : 0.0;
+            const double cum = start + kpp_block_scan(v, red);
+            if (j0 + t < r1b && cum >= target) atomicMin(&first, t);
             __syncthreads();
-            if (t == 0) {
-                found = 0;
-                double cum = base;
-                for (uint32_t i = 0; i < n; ++i) {
-                    if (cum + buf[i] >= target) {
-                        blk = j0 + i;
-                        found = 1;
-                        break;
-                    }
-                    cum += buf[i];
-                }
-                base = cum;
+            const uint32_t f = first;
+            if (f != 0xFFFFFFFFu) {
+                if (t == f) base = cum - v;
+                blk = j0 + f;
+                break;
             }
-            __syncthreads();
-            if (found) break;
+            if (t == kKppBlock - 1) base = cum;
         }
-        __syncthreads();
+        __syncthreads();  // base written, every read of `first` done
+        if (t == 0) first = 0xFFFFFFFFu;
         // the row inside that block
-        const uint32_t r0 = blk * 256, n = rows - r0 < 256 ? rows - r0 : 256;
-        if (t < n) buf[t] = (double)min_dist[r0 + t];
+        const uint32_t r0 = blk * kKppRows, n = rows - r0 < kKppRows ? rows - r0 : kKppRows;
+        const double start = base;
+        const double v = t < n ? (double)min_dist[r0 + t] : 0.0;
+        const double cum = start + kpp_block_scan(v, red);
+        if (t < n && cum >= target) atomicMin(&first, t);
         __syncthreads();
-        if (t == 0) {
-            double cum = base;
-            uint32_t pick = r0 + n - 1;
-            for (uint32_t i = 0; i < n; ++i) {
-                cum += buf[i];
-                if (cum >= target) {
-                    pick = r0 + i;
-                    break;
-                }
-            }
-            chosen = pick;
-        }
-        __syncthreads();
+        if (t == 0) chosen = first != 0xFFFFFFFFu ? r0 + first : r0 + n - 1;
     }
+    __syncthreads();
     copy_row(x, chosen, d4, dst);
+    if (lastb) {
+        const uint32_t row = chosen;
+        for (uint32_t k = t; k < d4; k += blockDim.x) {
+            const float4 a = x[((size_t)(row / 64) * d4 + k) * 64 + row % 64];
+            auto r = [](float f) { return (float)(__bf16)f; };
+            lastb[k] = make_float4(r(a.x), r(a.y), r(a.z), r(a.w));
+        }
+    }
 }
 
 __global__ void fill_f32_kernel(float *p, size_t n, float v) {
@@ -822,15 +848,17 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                                    1.01 * u * (1.0 + x_max * x_max)) *
                                   (1.0 + 1e-6);
         const uint4 *kpp_xb = (xbf16 && d4 % 2 == 0) ? xbf16 : nullptr;
+        // the choice of centroid c also writes its BF16 rounding (lastb) for
+        // pass c + 1
+        const uint32_t kb = (rows + kKppRows - 1) / kKppRows;
+        if (kpp_xb)
+            hipLaunchKernelGGL(bf16_round_row_kernel, dim3(1), dim3(64), 0, s, cent, d4, lastb);
         for (uint32_t c = 1; c < nlist; ++c) {
-            if (kpp_xb)
-                hipLaunchKernelGGL(bf16_round_row_kernel, dim3(1), dim3(64), 0, s,
-                                   cent + (size_t)(c - 1) * d4, d4, lastb);
-            hipLaunchKernelGGL(kpp_update_kernel, dim3(nb), dim3(256), 0, s, x, kpp_xb, rows, d4,
-                               cent + (size_t)(c - 1) * d4, lastb, kpp_margin, min_dist,
+            hipLaunchKernelGGL(kpp_update_kernel, dim3(kb), dim3(kKppRows), 0, s, x, kpp_xb, rows,
+                               d4, cent + (size_t)(c - 1) * d4, lastb, kpp_margin, min_dist,
                                partial);
-            hipLaunchKernelGGL(kpp_choose_kernel, dim3(1), dim3(256), 0, s, x, rows, d4, min_dist,
-                               partial, nb, rng, cent, c);
+            hipLaunchKernelGGL(kpp_choose_kernel, dim3(1), dim3(kKppBlock), 0, s, x, rows, d4,
+                               min_dist, partial, kb, rng, cent, c, kpp_xb ? lastb : nullptr);
         }
         NNDEX_HIP_CHECK(hipGetLastError());
 
