@@ -250,6 +250,20 @@ nndex_status nndex_gpu_index_build_ivf(nndex_gpu_index *index, nndex_ivf_info *o
 nndex_status nndex_gpu_index_ivf_export(nndex_gpu_index *index, float *centroids,
                                         uint64_t *offsets, uint64_t *original_indices,
                                         uint64_t *assignments);
+/* Install a saved IVF build (the arrays nndex_gpu_index_ivf_export writes,
+ * same layouts; assignments may be NULL) instead of building one, so an index
+ * re-created over the same rows skips the k-means build.  The reference keeps
+ * no saved form (ApproxIndex, src/approx.rs:11-25, is rebuilt lazily on each
+ * construction, src/gpu.rs:435-441); this is the persistence that build
+ * lacks.  Checks every array (offsets from 0 to rows and non-decreasing, a
+ * permutation, assignments consistent with both, finite centroids) and fails
+ * with NNDEX_ERR_INVALID_ARGUMENT if the index already has an IVF index.
+ * The reordered copy is gathered again from the resident rows; `out` (may be
+ * NULL) reports iters = 0 and build_ms = the import time. */
+nndex_status nndex_gpu_index_ivf_import(nndex_gpu_index *index, size_t nlist,
+                                        const float *centroids, const uint64_t *offsets,
+                                        const uint64_t *original_indices,
+                                        const uint64_t *assignments, nndex_ivf_info *out);
 /* Device-resident approximate search (candidate_indices + rerank,
  * src/approx.rs:110-138 and src/gpu.rs:448-462): the nprobe (0 = the
  * reference's choose_nprobe for k) clusters whose centroids score highest,

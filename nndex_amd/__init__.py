@@ -181,6 +181,69 @@ class NNdex:
             report += buf.value.decode()
         return report
 
+    # ---- IVF persistence (the reference rebuilds ApproxIndex on every
+    # construction, src/gpu.rs:435-441; these keep a GPU build) -----------
+
+    def _gpu_handle(self, what: str):
+        h = self.backend_handle
+        if h is None:
+            raise RuntimeError(f"{what} needs a GPU index")
+        return h
+
+    def ivf_export(self) -> dict:
+        """The IVF build of this index (built now if needed) as numpy arrays:
+        ``centroids`` [nlist, padded_dims] f32, ``offsets`` [nlist + 1],
+        ``original_indices`` [rows] (reordered position -> row) and
+        ``assignments`` [rows], all uint64 (ApproxIndex, src/approx.rs:11-25)."""
+        h = self._gpu_handle("ivf_export")
+        L = _lib.lib()
+        info = _lib.IvfInfo()
+        _lib.check(L.nndex_gpu_index_build_ivf(h, ctypes.byref(info)))
+        nl, pd = int(info.nlist), int(info.padded_dims)
+        cent = np.zeros((max(nl, 1), pd), np.float32)
+        off = np.zeros(nl + 1, np.uint64)
+        orig = np.zeros(self._rows, np.uint64)
+        assign = np.zeros(self._rows, np.uint64)
+        _lib.check(L.nndex_gpu_index_ivf_export(h, _fptr(cent), _u64ptr(off), _u64ptr(orig),
+                                                _u64ptr(assign)))
+        return {"centroids": cent[:nl], "offsets": off, "original_indices": orig,
+                "assignments": assign}
+
+    def ivf_import(self, centroids, offsets, original_indices, assignments=None) -> None:
+        """Install a saved IVF build (the arrays of ``ivf_export``) instead of
+        running the k-means build; the index must not have built one yet."""
+        h = self._gpu_handle("ivf_import")
+        cent = np.ascontiguousarray(centroids, np.float32)
+        if cent.ndim != 2 or cent.shape[1] != self.padded_dims:
+            raise ValueError(f"centroids must be [nlist, {self.padded_dims}] (padded dims)")
+        nl = cent.shape[0]
+        off = np.ascontiguousarray(offsets, np.uint64)
+        orig = np.ascontiguousarray(original_indices, np.uint64)
+        if off.shape != (nl + 1,) or orig.shape != (self._rows,):
+            raise ValueError("offsets must hold nlist + 1 values and original_indices rows")
+        asg = None
+        if assignments is not None:
+            asg = np.ascontiguousarray(assignments, np.uint64)
+            if asg.shape != (self._rows,):
+                raise ValueError("assignments must hold one value per row")
+        _lib.check(_lib.lib().nndex_gpu_index_ivf_import(
+            h, nl, _fptr(cent), _u64ptr(off), _u64ptr(orig),
+            _u64ptr(asg) if asg is not None else None, None))
+
+    def save_ivf(self, path: str) -> None:
+        """Write ``ivf_export()`` to an ``.npz`` file."""
+        np.savez(path, rows=np.uint64(self._rows), padded_dims=np.uint64(self.padded_dims),
+                 **self.ivf_export())
+
+    def load_ivf(self, path: str) -> None:
+        """Install an IVF build written by ``save_ivf`` for the same rows."""
+        with np.load(path) as f:
+            if int(f["rows"]) != self._rows or int(f["padded_dims"]) != self.padded_dims:
+                raise ValueError("saved IVF index was built for a different shape "
+                                 f"({int(f['rows'])} x {int(f['padded_dims'])})")
+            self.ivf_import(f["centroids"], f["offsets"], f["original_indices"],
+                            f["assignments"])
+
     @property
     def gpu_info(self) -> dict | None:
         info = _lib.GpuDeviceInfo()

@@ -175,6 +175,8 @@ SIGNATURES = {
     "nndex_gpu_index_set_ivf_params": (_ST, [_P, _SZ, _SZ, _SZ]),
     "nndex_gpu_index_build_ivf": (_ST, [_P, ctypes.POINTER(IvfInfo)]),
     "nndex_gpu_index_ivf_export": (_ST, [_P, _FP, _U64P, _U64P, _U64P]),
+    "nndex_gpu_index_ivf_import": (_ST, [_P, _SZ, _FP, _U64P, _U64P, _U64P,
+                                         ctypes.POINTER(IvfInfo)]),
     "nndex_gpu_index_search_ivf_device": (_ST, [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _P, _P]),
     "nndex_server_new": (_ST, [_P, ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
     "nndex_server_free": (None, [_P]),

@@ -1005,6 +1005,50 @@ const IvfIndex &GpuIndex::build_ivf() {
     return *ivf_;
 }
 
+const IvfIndex &GpuIndex::load_ivf(size_t nlist, const float *centroids, const uint64_t *offsets,
+                                   const uint64_t *orig, const uint64_t *assign) {
+    std::lock_guard<std::mutex> l(ivf_mu_);
+    if (ivf_) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "an IVF index already exists on this index");
+    if (!centroids || !offsets || !orig)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+    if (nlist < 2 || nlist > rows_ || nlist > 16384)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "nlist must be in [2, min(rows, 16384)]");
+    if (pdims_ > 1024)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF supports padded dims <= 1024");
+    for (size_t i = 0; i < nlist * pdims_; ++i)
+        if (!std::isfinite(centroids[i]))
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "IVF centroids must be finite");
+    if (offsets[0] != 0 || offsets[nlist] != rows_)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "IVF offsets must run from 0 to rows");
+    std::vector<uint32_t> off(nlist + 1), ord(rows_), asg(rows_);
+    for (size_t c = 0; c <= nlist; ++c) {
+        if (c && offsets[c] < offsets[c - 1])
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "IVF offsets must be non-decreasing");
+        off[c] = static_cast<uint32_t>(offsets[c]);
+    }
+    std::vector<uint8_t> seen(rows_, 0);
+    for (size_t c = 0; c < nlist; ++c)
+        for (size_t p = off[c]; p < off[c + 1]; ++p) {
+            const uint64_t r = orig[p];
+            if (r >= rows_ || seen[r])
+                throw Error(NNDEX_ERR_INVALID_ARGUMENT,
+                            "IVF original_indices must be a permutation of the rows");
+            if (assign && assign[r] != c)
+                throw Error(NNDEX_ERR_INVALID_ARGUMENT,
+                            "IVF assignments disagree with the offsets and original_indices");
+            seen[r] = 1;
+            ord[p] = static_cast<uint32_t>(r);
+            asg[r] = static_cast<uint32_t>(c);
+        }
+    NNDEX_HIP_CHECK(hipSetDevice(device_));
+    std::unique_ptr<IvfIndex> ivf(new IvfIndex());
+    ivf_load(*ivf, x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(pdims_),
+             static_cast<uint32_t>(nlist), centroids, off.data(), ord.data(), asg.data(), stream_);
+    ivf->cindex = from_device_rows(ivf->centroids, nlist, pdims_, device_);
+    ivf_ = std::move(ivf);
+    return *ivf_;
+}
+
 const IvfIndex *GpuIndex::approx_index() {
     if (!approx_) return nullptr;
     const IvfIndex &ivf = build_ivf();
@@ -1323,6 +1367,25 @@ nndex_status nndex_gpu_index_build_ivf(nndex_gpu_index *index, nndex_ivf_info *o
     return guard([&] {
         if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
         const IvfIndex &ivf = index->impl->build_ivf();
+        if (out) {
+            out->rows = ivf.rows;
+            out->padded_dims = ivf.pdims;
+            out->nlist = ivf.nlist;
+            out->iters = ivf.iters;
+            out->rechecked = ivf.rechecked;
+            out->build_ms = ivf.build_ms;
+        }
+    });
+}
+
+nndex_status nndex_gpu_index_ivf_import(nndex_gpu_index *index, size_t nlist,
+                                        const float *centroids, const uint64_t *offsets,
+                                        const uint64_t *original_indices,
+                                        const uint64_t *assignments, nndex_ivf_info *out) {
+    return guard([&] {
+        if (!index) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        const IvfIndex &ivf =
+            index->impl->load_ivf(nlist, centroids, offsets, original_indices, assignments);
         if (out) {
             out->rows = ivf.rows;
             out->padded_dims = ivf.pdims;

@@ -195,6 +195,10 @@ class GpuIndex {
     void set_ivf_params(size_t nlist, size_t nprobe, size_t iters);
     // The IVF index, built now if needed (any index).
     const IvfIndex &build_ivf();
+    // Install a saved IVF build instead of building one (checks the arrays;
+    // fails if an IVF index already exists).  assign may be null.
+    const IvfIndex &load_ivf(size_t nlist, const float *centroids, const uint64_t *offsets,
+                             const uint64_t *orig, const uint64_t *assign);
     // approx_index (src/gpu.rs:435-441): the IVF index for approximate
     // searches, built on first use, or nullptr when the index was created
     // without approx or IVF would not speed it up.

@@ -938,6 +938,41 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+void ivf_load(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uint32_t nlist,
+              const float *centroids, const uint32_t *offsets, const uint32_t *orig,
+              const uint32_t *assign, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t d4 = pdims / 4;
+    ivf.rows = rows;
+    ivf.pdims = pdims;
+    ivf.nlist = nlist;
+    ivf.iters = 0;
+    ivf.rechecked = 0;
+    const size_t prow = padded_rows(rows);
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.centroids, (size_t)nlist * pdims * sizeof(float)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.offsets, (nlist + 1) * sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.orig, (size_t)rows * sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.assign, (size_t)rows * sizeof(uint32_t)));
+    NNDEX_HIP_CHECK(hipMalloc(&ivf.xr, prow * d4 * sizeof(float4)));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(ivf.centroids, centroids, (size_t)nlist * pdims * sizeof(float),
+                                   hipMemcpyHostToDevice, s));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(ivf.offsets, offsets, (nlist + 1) * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(ivf.orig, orig, (size_t)rows * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+    NNDEX_HIP_CHECK(hipMemcpyAsync(ivf.assign, assign, (size_t)rows * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+    const size_t total = prow * d4;
+    uint32_t grid = (uint32_t)((total + 255) / 256);
+    if (grid > 65535u * 8) grid = 65535u * 8;
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3(grid), dim3(256), 0, s, x, rows, d4, ivf.orig,
+                       ivf.xr);
+    NNDEX_HIP_CHECK(hipGetLastError());
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+    ivf.build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 static uint32_t ivf_bits(size_t nlist) {
     uint32_t bits = 1;
     while ((1ull << bits) < nlist) ++bits;

@@ -82,6 +82,12 @@ size_t ivf_sort_bytes(size_t pairs, size_t nlist);
 void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows,
                uint32_t pdims, uint32_t nlist, uint32_t iters, double x_max, double x_err,
                uint32_t ncu, hipStream_t s);
+// Install a saved build (host arrays, already validated: offsets[0] = 0,
+// non-decreasing, offsets[nlist] = rows; orig a permutation; assign consistent
+// with both) and gather the reordered copy from the resident rows x.
+void ivf_load(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uint32_t nlist,
+              const float *centroids, const uint32_t *offsets, const uint32_t *orig,
+              const uint32_t *assign, hipStream_t s);
 // Scan the probed lists and merge per query (probe ids already on device).
 void ivf_scan(const IvfScanArgs &a, hipStream_t s);
 

@@ -214,3 +214,67 @@ def test_ivf_mixture_recall_and_structure(orc):
         rec = np.mean([len(set(aidx[i]) & set(eidx[i])) / k for i in range(len(qs))])
         print(f"nprobe {nprobe}: recall@10 {rec:.4f}", flush=True)
         assert rec >= floor
+
+
+def test_ivf_save_and_load_skip_the_build(orc, tmp_path):
+    """A saved GPU build installed on a new index over the same rows gives the
+    same arrays and bit-identical approximate results, without a k-means run
+    (the imported build reports 0 iterations); malformed arrays are refused
+    and leave the index able to import or build afterwards."""
+    _case("ivf_import")
+    import ctypes
+
+    from nndex_amd import NNdexStatusError, _lib
+
+    rng = np.random.default_rng(21)
+    centres = rng.normal(size=(30, 48)).astype(np.float32)
+    raw = centres[rng.integers(0, 30, size=5000)] + 0.3 * rng.normal(size=(5000, 48)).astype(
+        np.float32)
+    qs = rng.normal(size=(12, 48)).astype(np.float32)
+    a = nndex_amd.NNdex(raw, approx=True, backend="gpu", enable_cache=False)
+    _lib.check(_lib.lib().nndex_gpu_index_set_ivf_params(a.backend_handle, 40, 6, 5))
+    a_idx, a_sim = a.search(qs, k=10)
+    a_one = a.search(qs[0], k=7)
+    path = str(tmp_path / "ivf.npz")
+    a.save_ivf(path)
+    saved = a.ivf_export()
+
+    b = nndex_amd.NNdex(raw, approx=True, backend="gpu", enable_cache=False)
+    _lib.check(_lib.lib().nndex_gpu_index_set_ivf_params(b.backend_handle, 0, 6, 0))
+    bad = dict(saved)
+    bad["offsets"] = saved["offsets"].copy()
+    bad["offsets"][5], bad["offsets"][6] = bad["offsets"][6] + 1, bad["offsets"][5]
+    with pytest.raises(NNdexStatusError, match="non-decreasing"):
+        b.ivf_import(**bad)
+    bad = dict(saved)
+    bad["original_indices"] = saved["original_indices"].copy()
+    big = int(np.argmax(np.diff(saved["offsets"].astype(np.int64))))
+    p0 = int(saved["offsets"][big])  # two positions of one cluster
+    bad["original_indices"][p0] = bad["original_indices"][p0 + 1]
+    with pytest.raises(NNdexStatusError, match="permutation"):
+        b.ivf_import(**bad)
+    bad = dict(saved)
+    bad["assignments"] = (saved["assignments"] + 1) % 40
+    with pytest.raises(NNdexStatusError, match="disagree"):
+        b.ivf_import(**bad)
+    b.load_ivf(path)
+    info = _lib.IvfInfo()
+    _lib.check(_lib.lib().nndex_gpu_index_build_ivf(b.backend_handle, ctypes.byref(info)))
+    assert int(info.iters) == 0 and int(info.nlist) == 40
+    got = b.ivf_export()
+    for key in saved:
+        assert np.array_equal(got[key], saved[key]), key
+    b_idx, b_sim = b.search(qs, k=10)
+    np.testing.assert_array_equal(b_idx, a_idx)
+    assert np.array_equal(b_sim.view(np.uint32), a_sim.view(np.uint32))
+    b_one = b.search(qs[0], k=7)
+    np.testing.assert_array_equal(b_one[0], a_one[0])
+    assert np.array_equal(b_one[1].view(np.uint32), a_one[1].view(np.uint32))
+    with pytest.raises(NNdexStatusError, match="already exists"):
+        b.load_ivf(path)
+    # the oracle's approximate search over the same build agrees
+    m = orc.normalize_rows(raw, 5000, 48)
+    ref = orc.Ivf(m, nlist=40, iters=5)
+    ridx, rsim = ref.search_gpu(orc.normalize_vector(qs[0]), 7, 6)
+    np.testing.assert_array_equal(b_one[0], ridx)
+    assert np.array_equal(b_one[1].view(np.uint32), rsim.view(np.uint32))
