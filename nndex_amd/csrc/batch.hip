@@ -377,7 +377,8 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     const uint32_t block_rows = NW * RW;
     const uint32_t nblk = (a.rows_padded + block_rows - 1) / block_rows;
     const uint32_t nchunks = (a.nq + QC - 1) / QC;
-    const uint32_t chunk_units = QC * dch;
+    constexpr uint32_t RU = 4 * MAXS;    // staged units per query (>= dch)
+    constexpr uint32_t chunk_units = QC * RU;
     if (a.nq == 0 || dch == 0) return;
     auto thl = [&](uint32_t buf) {
         return reinterpret_cast<unsigned long long *>(qlds + buf * buf_units + q_units);
@@ -416,12 +417,16 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
         a.mode == 0 ? a.thresh : reinterpret_cast<const unsigned long long *>(a.q);
     uint4 stg[STG];
     unsigned long long stg_th;
+    // Staging index i covers query n = i / RU, unit c = i % RU, with RU =
+    // 4 * MAXS a compile-time count (units c >= dch are zero), so the split
+    // is a multiply and shift rather than a division by the run-time width.
     auto fetch = [&](uint32_t ch) {
 #pragma unroll
         for (int j = 0; j < STG; ++j) {
             const uint32_t i = tid + j * NTH;
             const uint32_t ic = i < chunk_units ? i : chunk_units - 1;
-            const uint32_t n = ic / dch, c = ic - n * dch;
+            const uint32_t n = ic / RU, c0 = ic - n * RU;
+            const uint32_t c = c0 < dch ? c0 : dch - 1;
             uint32_t qi = ch * QC + n;
             qi = qi < a.nq ? qi : a.nq - 1;
             stg[j] = a.q[(size_t)qi * a.q_stride + c];
@@ -436,9 +441,10 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
         for (int j = 0; j < STG; ++j) {
             const uint32_t i = tid + j * NTH;
             if (i < chunk_units) {
-                const uint32_t n = i / dch, c = i - n * dch;
-                dst[c * kStride + ((n & ~15u) | ((n ^ c) & 15u))] =
-                    ch * QC + n < a.nq ? stg[j] : make_uint4(0, 0, 0, 0);
+                const uint32_t n = i / RU, c = i - n * RU;
+                if (c < dch)
+                    dst[c * kStride + ((n & ~15u) | ((n ^ c) & 15u))] =
+                        ch * QC + n < a.nq ? stg[j] : make_uint4(0, 0, 0, 0);
             }
         }
         if (tid < QC)

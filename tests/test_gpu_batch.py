@@ -225,3 +225,37 @@ def test_batch_tile_variants_agree_and_autotune(dims):
     for i in range(300):
         if not fl[i]:
             assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
+
+
+@pytest.mark.parametrize("dims", [600, 768, 1000])
+def test_wide_rows_tile_variants(dims):
+    """Above 512 dims (BF16 and FP8 scoring only) with either tile variant:
+    every certified BF16 query equals the exact FP32 search bit for bit, and
+    FP8 returns exact FP32 scores of the rows it finds, with recall@10 near 1
+    on the mixture."""
+    _case("wide_rows", dims=dims)
+    base, queries = synth.mixture_workload(rows=40_000, dims=dims, n_queries=120)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 120, 64)
+    L = _lib.lib()
+    ei, es = exact_device(b, ds, queries, 10)
+    for prec in (1, 2):
+        for v in (0, 1):
+            check(L.nndex_gpu_index_set_batch_tiles(b.h, prec, v))
+            gi, gs, fl = batch_device(b, ds, queries, 10, prec)
+            if prec == 1:
+                ok = fl == 0
+                assert ok.mean() > 0.9, (prec, v, ok.mean())
+                for i in np.flatnonzero(ok):
+                    assert gi[i].tolist() == ei[i].tolist(), (prec, v, i)
+                    assert gs[i].tolist() == es[i].tolist(), (prec, v, i)
+            else:
+                hits = sum(len(set(gi[i].tolist()) & set(ei[i].tolist())) for i in range(120))
+                assert hits / 1200 > 0.99, (v, hits)
+                exact = {(i, int(r)): s for i in range(120) for r, s in zip(ei[i], es[i])}
+                for i in range(120):
+                    for r, s in zip(gi[i], gs[i]):
+                        if (i, int(r)) in exact:
+                            assert s == exact[(i, int(r))], (v, i, int(r))
+    check(L.nndex_gpu_index_set_batch_tiles(b.h, 1, 0))
+    check(L.nndex_gpu_index_set_batch_tiles(b.h, 2, 0))
