@@ -695,8 +695,9 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Sample threshold: per query, the k'-th best of `n` sample scores as a key
-// threshold (keys >= thresh pass).  n < k' -> 0 (all pass).
+// Sample threshold: per query, the kp-th best of `n` sample scores as a key
+// threshold (keys >= thresh pass; the caller picks the rank kp so that a few
+// times the rescored count of rows pass).  n < kp -> 0 (all pass).
 // ---------------------------------------------------------------------------
 template <int CW>
 __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__restrict__ scores,
@@ -740,7 +741,7 @@ __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__re
             }
         tk.compact();
         if (lane == 0) {
-            // At least k' sample rows score >= t; rows under t are bounded by
+            // At least kp sample rows score >= t; rows under t are bounded by
             // t + delta in the certificate, so no lowering is needed here.
             const float lowered = key_sim(slots[0][kp - 1]);
             thresh[q] = (unsigned long long)ordered_bits(lowered) << 32;

@@ -348,7 +348,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         };
         launch_mfma_scan(a, elem, grid_for(a.rows_padded), s, variant);
         launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, a.ldo,
-                                static_cast<uint32_t>(kp), ws.delta, ws.thresh, s);
+                                static_cast<uint32_t>(plan.jt), ws.delta, ws.thresh, s);
         // stage 2: filter scan over every row
         launch_zero_u32(ws.cand_count, n, s);
         a.x = bf16 ? xbf16_ : (fp8 ? xfp8_ : reinterpret_cast<const uint4 *>(x_));
@@ -428,6 +428,9 @@ BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
     if (stride > 64) stride = 64;
     b.stride = static_cast<uint32_t>(stride);
     b.rows_s = (rows_ + stride - 1) / stride;
+    size_t jt = (2 * (kp + 1) + stride - 1) / stride;
+    if (jt < 8) jt = 8;
+    b.jt = jt < kp ? jt : kp;
     b.capq = 8 * kp * stride;
     if (b.capq < 8192) b.capq = 8192;
     return b;

@@ -78,6 +78,13 @@ struct Workspace {
 // sample stride and rows, candidate capacity per query, queries per group.
 struct BatchPlan {
     size_t kp = 0;
+    // rank of the filter threshold among the sample scores: about 2 (k' + 1)
+    // rows of the index score at or above it (at least 8 sample rows), so
+    // the k' + 1 best approximate scores are almost always among the
+    // candidates (when fewer pass, the certificate bounds the rest by the
+    // threshold itself); each candidate costs the scan an append and a
+    // global atomic, so fewer candidates make the scan faster
+    size_t jt = 0;
     uint32_t stride = 1;
     size_t rows_s = 0;
     size_t capq = 0;
