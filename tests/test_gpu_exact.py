@@ -331,3 +331,35 @@ def test_frontend_autotune_keeps_results(orc):
     after = index.search(qs, k=10), index.search(qs[0], k=10)
     for (bi, bs), (ai, as_) in zip(before, after):
         assert np.array_equal(bi, ai) and np.array_equal(bs, as_)
+
+
+# ---------------------------------------------------------------------------
+# Committed fixtures (tests/golden/fixtures.npz)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["dims37", "ties", "negative", "k_gt_rows"])
+def test_committed_fixtures_through_frontend(name):
+    """Single queries, a batch of 6 (fused scan) and the same queries twice
+    (12: the certified MFMA path) equal the committed reference-path results
+    bit for bit; scores agree with the committed FP64 values to 1e-5."""
+    import os
+
+    _print_case(f"fixture_{name}")
+    path = os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz")
+    with np.load(path) as f:
+        fx = {key: f[key] for key in f.files if key.startswith(name + "/")}
+    matrix, queries = fx[f"{name}/matrix"], fx[f"{name}/queries"]
+    k = int(fx[f"{name}/k"])
+    index = nndex_amd.NNdex(matrix, normalized=False, backend="gpu", enable_cache=False)
+    for i in range(queries.shape[0]):
+        idx, sim = index.search(queries[i], k=k)
+        assert idx.tolist() == fx[f"{name}/single_idx"][i].tolist(), i
+        assert np.array_equal(sim.view(np.uint32), fx[f"{name}/single_sim"][i].view(np.uint32)), i
+        np.testing.assert_allclose(sim, fx[f"{name}/fp64_sim"][i], atol=1e-5, rtol=0)
+    for qs in (queries, np.vstack([queries, queries])):
+        bidx, bsim = index.search(qs, k=k)
+        for i in range(qs.shape[0]):
+            j = i % queries.shape[0]
+            assert bidx[i].tolist() == fx[f"{name}/batch_idx"][j].tolist(), (qs.shape, i)
+            assert np.array_equal(bsim[i].view(np.uint32),
+                                  fx[f"{name}/batch_sim"][j].view(np.uint32)), (qs.shape, i)

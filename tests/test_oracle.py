@@ -493,3 +493,63 @@ def test_accuracy_vs_numpy_exact(orc):
                                  orc.normalize_vector(query), k)
     assert np.array_equal(idx.astype(np.int64), np_idx)
     assert np.allclose(sim, scores[np_idx], atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# Committed fixtures (tests/golden/make_fixtures.py): FP64 brute force from
+# numpy beside the oracle's restatement of the reference paths
+# ---------------------------------------------------------------------------
+
+def _fixtures():
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz")
+    with np.load(path) as f:
+        return {k: f[k] for k in f.files}
+
+
+FIXTURE_CASES = ["dims37", "ties", "negative", "k_gt_rows"]
+
+
+def test_fixture_fp64_matches_the_published_golden_vector():
+    fx = _fixtures()
+    assert fx["golden50k/fp64_idx"][0].tolist() == synth.GOLDEN_50K_INDICES
+    np.testing.assert_allclose(fx["golden50k/fp64_sim"][0], synth.GOLDEN_50K_SCORES,
+                               atol=1e-5, rtol=0)
+    assert fx["golden50k/fp64_idx"].shape == (5, 5)
+
+
+@pytest.mark.parametrize("name", FIXTURE_CASES)
+def test_oracle_reproduces_fixtures_and_agrees_with_fp64(orc, name):
+    """The oracle's reference-path results equal the committed ones bit for
+    bit, and agree with the independent FP64 brute force: identical scores to
+    1e-5 position by position, identical indices except among rows whose FP64
+    scores tie (duplicates keep the reference heap's choice)."""
+    fx = _fixtures()
+    matrix, queries = fx[f"{name}/matrix"], fx[f"{name}/queries"]
+    k = int(fx[f"{name}/k"])
+    rows, dims = matrix.shape
+    m = orc.normalize_rows(matrix, rows, dims)
+    qb = orc.normalize_rows(queries, queries.shape[0], dims)
+    f_idx, f_sim = fx[f"{name}/fp64_idx"], fx[f"{name}/fp64_sim"]
+    assert f_idx.shape[1] == min(k, rows)
+    for i in range(queries.shape[0]):
+        qs = orc.normalize_vector(queries[i])
+        s_idx, s_sim = orc.gpu_exact_search(m, qs, k)
+        b_idx, b_sim = orc.gpu_exact_search(m, qb[i], k)
+        assert s_idx.tolist() == fx[f"{name}/single_idx"][i].tolist()
+        assert np.array_equal(s_sim.view(np.uint32), fx[f"{name}/single_sim"][i].view(np.uint32))
+        assert b_idx.tolist() == fx[f"{name}/batch_idx"][i].tolist()
+        assert np.array_equal(b_sim.view(np.uint32), fx[f"{name}/batch_sim"][i].view(np.uint32))
+        assert orc.cpu_search(m, qs, k)[0].tolist() == fx[f"{name}/cpu_idx"][i].tolist()
+        np.testing.assert_allclose(s_sim, f_sim[i], atol=1e-5, rtol=0)
+        for j, (a, b) in enumerate(zip(s_idx.tolist(), f_idx[i].tolist())):
+            if a != b:  # only among FP64-tied rows
+                full = np.asarray(matrix, np.float64)
+                full /= np.linalg.norm(full, axis=1, keepdims=True)
+                q64 = queries[i].astype(np.float64) / np.linalg.norm(queries[i].astype(np.float64))
+                assert abs(full[a] @ q64 - full[b] @ q64) < 1e-12, (name, i, j)
+    if name == "negative":
+        assert (f_sim < 0).all()
+    if name == "ties":
+        assert any(fx["ties/single_idx"][i].tolist() != f_idx[i].tolist() for i in range(6))
