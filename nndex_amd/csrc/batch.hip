@@ -477,7 +477,14 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 }
             if (__ballot(!(mx < thf)) == 0) continue;
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+            for (int m = 0; m < MT; ++m) {
+                // the same test per 16-row tile: most passing tiles hold one
+                // pair, so the per-row loop below runs for one row tile
+                if constexpr (MT > 1) {
+                    const float mm = __builtin_fmaxf(__builtin_fmaxf(sc[m][0], sc[m][1]),
+                                                     __builtin_fmaxf(sc[m][2], sc[m][3]));
+                    if (__ballot(!(mm < thf)) == 0) continue;
+                }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
@@ -497,6 +504,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                     cq_n += n;
                 }
+            }
         }
     };
     // Half of the waves (one of each SIMD's pair) run the mode 0 epilogue one
