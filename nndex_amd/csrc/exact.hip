@@ -506,7 +506,9 @@ uint32_t scan_max_qb(uint32_t k) {
     uint32_t cw = scan_slots_for_k(k);
     if (cw > 2048) return 0;
     uint32_t qb = 2048 / cw;  // 4 waves x QB x CW x 8 B <= 64 KiB
-    return qb > 8 ? 8 : qb;
+    // at most 4 queries per launch: eight run out of scalar registers for
+    // their query chunks and take longer than two launches of four
+    return qb > 4 ? 4 : qb;
 }
 
 void launch_exact_scan(const ScanArgs &a, uint32_t qb, uint32_t grid, hipStream_t s) {
@@ -528,8 +530,6 @@ void launch_exact_scan(const ScanArgs &a, uint32_t qb, uint32_t grid, hipStream_
     NNDEX_SCAN_CASE(4, 128)
     NNDEX_SCAN_CASE(4, 256)
     NNDEX_SCAN_CASE(4, 512)
-    NNDEX_SCAN_CASE(8, 128)
-    NNDEX_SCAN_CASE(8, 256)
 #undef NNDEX_SCAN_CASE
     throw Error(NNDEX_ERR_UNSUPPORTED, "no exact scan variant for qb=" + std::to_string(qb) +
                                            " k=" + std::to_string(a.k));
