@@ -16,8 +16,12 @@ Also reported:
                 events' own overhead);
   cpu_baseline  the CPU oracle's restatement of the reference exact path
                 (ParallelFold strategy, all threads) on a bounded sample;
-  batch         QPS at batch sizes 16 / 256 / 4096 (exact FP32) and the BF16
-                scoring + exact FP32 rerank form at 4096;
+  batch         QPS at batch sizes 16 / 256 / 4096 of exact search with FP32
+                MFMA scoring (fp32_b*: certificate, then the fused scan for
+                open queries) and of the default exact path (exact_b*: BF16
+                scoring where the row width allows, same certificate and
+                fallback), and the BF16 scoring + exact FP32 rerank form at
+                4096 against fp32_b4096;
   precision     the precision form: 5,000,000 x 768 anisotropic base (x_i ~
                 N(0,1) * i^-0.5, unit-normalised), 1,024 queries, k=100: the
                 exact search against BF16 and FP8-E4M3 (per-row scale)
@@ -560,7 +564,7 @@ def main():
     host_qps = nh / (time.perf_counter() - t0)
     log(f"host API: {host_qps:.0f} QPS")
 
-    # ---- batches (exact FP32) ----
+    # ---- batches: exact search, FP32 scoring and the default path ----
     batch = {}
     batch_tune = None
     if not args.no_batch and not args.no_autotune:
@@ -571,18 +575,34 @@ def main():
         log("batch autotune:\n" + "\n".join(tuned))
         batch_tune = [ln for ln in tuned if ln.startswith("batch ")]
     if not args.no_batch:
+        xfl = Dev(4096)
+
+        def fp32_exact(B):
+            # exact search with FP32 MFMA scoring: the certificate, then the
+            # fused scan for every query it leaves open
+            check(L.nndex_gpu_index_search_batch_device(h, ws, dq.ptr, B, k, 0, 0, didx.ptr,
+                                                        dsim.ptr, xfl.ptr, stream))
+            check(L.nndex_stream_synchronize(stream))
+            for i in np.flatnonzero(xfl.get((B,), np.uint8)):
+                check(L.nndex_gpu_index_search_device(h, ws, dq.at(int(i) * qbytes), 1, k,
+                                                      didx.at(int(i) * k * 8),
+                                                      dsim.at(int(i) * k * 4), stream))
+
         for B in (16, 256, 4096):
             reps = max(2, min(50, int(2000 // B)))
-            step(0, B)
-            check(L.nndex_stream_synchronize(stream))
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                check(L.nndex_gpu_index_search_device(h, ws, dq.ptr, B, k, didx.ptr, dsim.ptr,
-                                                      stream))
-            check(L.nndex_stream_synchronize(stream))
-            dt = (time.perf_counter() - t0) / reps
-            batch[f"fp32_b{B}"] = {"qps": round(B / dt, 1), "ms_per_batch": round(dt * 1e3, 3)}
-            log(f"batch B={B}: {B / dt:.0f} QPS ({dt * 1e3:.2f} ms)")
+            for name, run in (("fp32", lambda: fp32_exact(B)),
+                              ("exact", lambda: check(L.nndex_gpu_index_search_device(
+                                  h, ws, dq.ptr, B, k, didx.ptr, dsim.ptr, stream)))):
+                run()
+                check(L.nndex_stream_synchronize(stream))
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    run()
+                check(L.nndex_stream_synchronize(stream))
+                dt = (time.perf_counter() - t0) / reps
+                batch[f"{name}_b{B}"] = {"qps": round(B / dt, 1),
+                                         "ms_per_batch": round(dt * 1e3, 3)}
+                log(f"batch {name} B={B}: {B / dt:.0f} QPS ({dt * 1e3:.2f} ms)")
 
     # ---- BF16 scoring + exact FP32 rerank at B=4096 vs exact FP32 ----
     if not args.no_batch:

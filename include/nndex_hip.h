@@ -140,10 +140,11 @@ void nndex_workspace_free(nndex_workspace *ws);
 
 /* Exact FP32 search of `query_rows` normalised padded queries at
  * `device_queries`; writes query_rows x k (index, similarity) to the device
- * outputs (k <= rows, k <= the workspace's max_k).  Up to 8 queries run the
+ * outputs (k <= rows, k <= the workspace's max_k).  Up to 4 queries run the
  * fused scan without synchronising; larger batches run the certified MFMA
- * path and synchronise `stream` once to re-run any uncertified query with
- * the fused scan, so every result is exact. */
+ * path (BF16 scoring up to 1024 padded dims, else FP32) and synchronise
+ * `stream` once to re-run any uncertified query with the fused scan, so every
+ * result is exact. */
 nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspace *ws,
                                            const float *device_queries, size_t query_rows,
                                            size_t k, uint64_t *device_indices,
@@ -270,7 +271,7 @@ nndex_status nndex_gpu_index_ivf_import(nndex_gpu_index *index, size_t nlist,
  * then every row of those clusters rescored exactly into a top-k.  Writes
  * query_rows x k results and device_lens[q] = results found (< k when the
  * probed clusters hold fewer than k rows).  Builds the IVF index first if
- * needed.  Synchronises `stream` once when query_rows > 8. */
+ * needed.  Synchronises `stream` once when query_rows > 4. */
 nndex_status nndex_gpu_index_search_ivf_device(nndex_gpu_index *index, nndex_workspace *ws,
                                                const float *device_queries, size_t query_rows,
                                                size_t k, size_t nprobe,

@@ -76,6 +76,10 @@ struct Workspace {
 
 // Sizes of the batched MFMA path for one (k, precision): over-fetch k',
 // sample stride and rows, candidate capacity per query, queries per group.
+// Exact batches of up to this many queries run the fused scan (one launch);
+// larger ones the certified MFMA path, which is faster from five queries on.
+constexpr size_t kFusedBatchMax = 4;
+
 struct BatchPlan {
     size_t kp = 0;
     // rank of the filter threshold among the sample scores: about 2 (k' + 1)
@@ -186,8 +190,9 @@ class GpuIndex {
     bool mfma_supported(Precision p) const;
     static size_t default_overfetch(size_t k, Precision p);
     BatchPlan plan_batch(size_t k, Precision p, size_t kp) const;
-    // Precision of the exact batch path (FP32 MFMA up to 512 padded dims,
-    // BF16 above); false when neither MFMA form applies (fused scan only).
+    // Scoring precision of the exact batch path (BF16 up to 1024 padded
+    // dims, else FP32 up to 512; the certificate and the scan fallback make
+    // either exact); false when neither MFMA form applies (fused scan only).
     bool exact_batch_precision(Precision &p) const;
     // Allocate everything a later search of up to nq queries and k results
     // needs (lazy copies, row sample, workspace buffers; IVF index and its

@@ -259,7 +259,8 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
     uint32_t kstride = k;  // result row stride of this batch
     bool graphed = false;
     Precision prec;
-    if (mode == 0 && opt_.use_graphs && nq > 8 && index_->exact_batch_precision(prec)) {
+    if (mode == 0 && opt_.use_graphs && nq > kFusedBatchMax &&
+        index_->exact_batch_precision(prec)) {
         // replay a captured exact batch: rows padded to a power of two (with
         // copies of the last row), k to a power of two (results are prefixes)
         uint32_t qb = 16, kb = 1;

@@ -156,7 +156,9 @@ def test_batch_position_independence_mfma():
     perm = rng.permutation(200)
     i2, s2 = exact_device(b, ds, qs[perm], 10)
     assert np.array_equal(i1[perm], i2) and np.array_equal(s1[perm], s2)
-    i3, s3 = exact_device(b, ds, qs[:5], 10)  # scan path for the first five
+    i3, s3 = exact_device(b, ds, qs[:4], 10)  # fused scan path for the first four
+    assert np.array_equal(i1[:4], i3) and np.array_equal(s1[:4], s3)
+    i3, s3 = exact_device(b, ds, qs[:5], 10)  # a five-query MFMA batch
     assert np.array_equal(i1[:5], i3) and np.array_equal(s1[:5], s3)
 
 
