@@ -421,8 +421,9 @@ BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
     if (kp > rows_) kp = rows_;
     if (kp > 447) throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 447 is not supported");
     b.kp = kp;
-    // sample: about max(256 k', rows/16) rows -> about 16 k' candidates
-    const size_t target = 256 * kp > rows_ / 16 ? 256 * kp : rows_ / 16;
+    // sample: about max(128 k', rows/32) rows; the threshold's rank in it
+    // (below) sets how many rows pass the filter
+    const size_t target = 128 * kp > rows_ / 32 ? 128 * kp : rows_ / 32;
     size_t stride = rows_ / (target ? target : 1);
     if (stride < 1) stride = 1;
     if (stride > 64) stride = 64;
