@@ -443,14 +443,17 @@ size_t BatchPlan::workspace_bytes(size_t nq, size_t pdims) const {
 }
 
 bool GpuIndex::exact_batch_precision(Precision &p) const {
-    // BF16 scoring first: its certificate proves most queries' results equal
-    // the FP32 search (the rest take the fused scan), at a quarter of the
-    // FP32 MFMA time and half the bytes
-    if (mfma_supported(Precision::BF16)) {
+    // BF16 scoring for large indexes: its certificate proves most queries'
+    // results equal the FP32 search (the rest take the fused scan), at a
+    // quarter of the FP32 MFMA time and half the bytes.  Small indexes (the
+    // IVF centroid index among them) keep FP32, whose tighter bound leaves
+    // fewer queries to the scan where the scoring itself is cheap.
+    const bool f32 = mfma_supported(Precision::F32);
+    if (mfma_supported(Precision::BF16) && (rows_ >= kBf16ExactMinRows || !f32)) {
         p = Precision::BF16;
         return true;
     }
-    if (mfma_supported(Precision::F32)) {
+    if (f32) {
         p = Precision::F32;
         return true;
     }

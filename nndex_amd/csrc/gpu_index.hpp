@@ -79,6 +79,8 @@ struct Workspace {
 // Exact batches of up to this many queries run the fused scan (one launch);
 // larger ones the certified MFMA path, which is faster from five queries on.
 constexpr size_t kFusedBatchMax = 4;
+// Exact batches over at least this many rows score on BF16 (FP32 below).
+constexpr size_t kBf16ExactMinRows = 65536;
 
 struct BatchPlan {
     size_t kp = 0;
@@ -190,9 +192,10 @@ class GpuIndex {
     bool mfma_supported(Precision p) const;
     static size_t default_overfetch(size_t k, Precision p);
     BatchPlan plan_batch(size_t k, Precision p, size_t kp) const;
-    // Scoring precision of the exact batch path (BF16 up to 1024 padded
-    // dims, else FP32 up to 512; the certificate and the scan fallback make
-    // either exact); false when neither MFMA form applies (fused scan only).
+    // Scoring precision of the exact batch path (BF16 up to 1024 padded dims
+    // for indexes of kBf16ExactMinRows rows or more, else FP32 up to 512; the
+    // certificate and the scan fallback make either exact); false when
+    // neither MFMA form applies (fused scan only).
     bool exact_batch_precision(Precision &p) const;
     // Allocate everything a later search of up to nq queries and k results
     // needs (lazy copies, row sample, workspace buffers; IVF index and its

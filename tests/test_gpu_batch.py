@@ -51,6 +51,8 @@ def exact_device(b: BackendIndex, ds: DeviceSearch, qs: np.ndarray, k: int):
     (30_000, 96, 64, 100),
     (3_000, 8, 33, 20),
     (12_000, 768, 24, 10),   # above 512 padded dims: BF16 scoring, certified or rescanned
+    (100_000, 128, 40, 10),  # 65,536 rows or more: BF16 scoring for exact batches
+    (70_000, 64, 6, 3),      # five queries or more take the MFMA path
 ])
 def test_exact_batch_through_frontend(orc, rows, dims, nq, k):
     """Batches above 8 queries take the certified MFMA path; every result must
