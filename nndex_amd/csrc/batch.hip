@@ -353,8 +353,8 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 constexpr uint32_t kCandQueue = 128;
 
 // Query buffers of the MFMA scan in LDS (16-byte units each) and how many
-// fit: four where 160 KiB allow (chunks staged two ahead, one barrier per
-// two chunks), else two (one ahead, a barrier per chunk).
+// fit in 160 KiB beside the candidate queues: 2P buffers stage chunks P
+// ahead with one barrier per P chunks (P = 4, 3, 2 or 1).
 template <int NW, int MAXS, int NT, bool AM>
 constexpr uint32_t mfma_buf_units() {
     return (uint32_t)(MAXS * 4 * 16 * NT + 8 * NT);
@@ -362,7 +362,8 @@ constexpr uint32_t mfma_buf_units() {
 template <int NW, int MAXS, int NT, bool AM>
 constexpr uint32_t mfma_nbuf() {
     constexpr size_t queue = AM ? 0 : (size_t)NW * kCandQueue * 12;
-    return 4 * (size_t)mfma_buf_units<NW, MAXS, NT, AM>() * 16 + queue <= 160 * 1024 ? 4 : 2;
+    constexpr size_t b = (size_t)mfma_buf_units<NW, MAXS, NT, AM>() * 16, lds = 160 * 1024;
+    return 8 * b + queue <= lds ? 8 : (6 * b + queue <= lds ? 6 : (4 * b + queue <= lds ? 4 : 2));
 }
 
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM>
