@@ -352,20 +352,6 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // Entries of a wave's LDS candidate queue (12 bytes each).
 constexpr uint32_t kCandQueue = 128;
 
-// Query buffers of the MFMA scan in LDS (16-byte units each) and how many
-// fit in 160 KiB beside the candidate queues: 2P buffers stage chunks P
-// ahead with one barrier per P chunks (P = 4, 3, 2 or 1).
-template <int NW, int MAXS, int NT, bool AM>
-constexpr uint32_t mfma_buf_units() {
-    return (uint32_t)(MAXS * 4 * 16 * NT + 8 * NT);
-}
-template <int NW, int MAXS, int NT, bool AM>
-constexpr uint32_t mfma_nbuf() {
-    constexpr size_t queue = AM ? 0 : (size_t)NW * kCandQueue * 12;
-    constexpr size_t b = (size_t)mfma_buf_units<NW, MAXS, NT, AM>() * 16, lds = 160 * 1024;
-    return 8 * b + queue <= lds ? 8 : (6 * b + queue <= lds ? 6 : (4 * b + queue <= lds ? 4 : 2));
-}
-
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM>
 __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     static_assert(!(AM && E::SCALED), "the row argmax takes unscaled scores");
@@ -379,7 +365,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     constexpr uint32_t NTH = NW * 64;
     constexpr int STG = (int)((QC * 4 * MAXS + NTH - 1) / NTH);  // staged units per thread
     constexpr uint32_t TU = QC / 2;      // threshold units (two 64-bit keys each) per buffer
-    extern __shared__ uint4 qlds[];      // NBUF buffers x ([4*steps units][kStride] + TU)
+    extern __shared__ uint4 qlds[];      // 2 buffers x ([4*steps units][kStride] + TU)
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t wid = tid >> 6;
@@ -397,15 +383,12 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     auto thl = [&](uint32_t buf) {
         return reinterpret_cast<unsigned long long *>(qlds + buf * buf_units + q_units);
     };
-    // Mode 0 candidate queue of the wave in LDS (after the buffers): keys
+    // Mode 0 candidate queue of the wave in LDS (after both buffers): keys
     // and query ids of the pairs that passed, flushed to the per-query lists
     // when full and at the end of each row tile, so the epilogue never waits
     // on a returning atomic.
-    constexpr uint32_t NBUF = mfma_nbuf<NW, MAXS, NT, AM>();
-    constexpr uint32_t PD_CH = NBUF / 2;  // chunks staged ahead; barrier every PD_CH chunks
-    static_assert(buf_units == mfma_buf_units<NW, MAXS, NT, AM>(), "LDS layout");
-    uint64_t *cq_key = reinterpret_cast<uint64_t *>(qlds + NBUF * buf_units) + wid * kCandQueue;
-    uint32_t *cq_qi = reinterpret_cast<uint32_t *>(qlds + NBUF * buf_units + NW * kCandQueue / 2) +
+    uint64_t *cq_key = reinterpret_cast<uint64_t *>(qlds + 2 * buf_units) + wid * kCandQueue;
+    uint32_t *cq_qi = reinterpret_cast<uint32_t *>(qlds + 2 * buf_units + NW * kCandQueue / 2) +
                       wid * kCandQueue;
     uint32_t cq_n = 0;  // wave-uniform
     auto flush = [&]() {
@@ -421,7 +404,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     };
 
     // zero the padding units of both buffers once (units >= dch)
-    for (uint32_t i = tid; i < NBUF * buf_units; i += NTH) {
+    for (uint32_t i = tid; i < 2 * buf_units; i += NTH) {
         const uint32_t r = i % buf_units;
         if (r < q_units && r / kStride >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
     }
@@ -571,20 +554,14 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 }
             }
         __syncthreads();  // previous block tile done with both buffers
-        for (uint32_t j = 0; j < PD_CH && j < nchunks; ++j) {
-            fetch(j);
-            commit(j, j);
-        }
+        fetch(0);
+        commit(0, 0);
         __syncthreads();
-        // Chunk ch reads buffer ch % NBUF; chunk ch + PD_CH is staged during
-        // chunk ch into the buffer chunk ch - PD_CH used.  A barrier after
-        // every PD_CH-th chunk orders each commit before its chunk's reads
-        // and after the previous reads of its buffer.
         for (uint32_t ch = 0; ch < nchunks; ++ch) {
-            const bool more = ch + PD_CH < nchunks;
-            fetch(more ? ch + PD_CH : ch);
-            const unsigned long long *thc = thl(ch % NBUF);
-            const uint4 *src = qlds + (ch % NBUF) * buf_units;
+            const bool more = ch + 1 < nchunks;
+            fetch(more ? ch + 1 : ch);
+            const unsigned long long *thc = thl(ch & 1);
+            const uint4 *src = qlds + (ch & 1) * buf_units;
             if (late && ch > 0) epi0(accp, thkp, ch - 1, bt, rs);
             v4f acc[MT][NT][E::NACC];
 #pragma unroll
@@ -688,8 +665,8 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 }
             }
-            if (more) commit((ch + PD_CH) % NBUF, ch + PD_CH);
-            if (ch % PD_CH == PD_CH - 1) __syncthreads();
+            if (more) commit((ch + 1) & 1, ch + 1);
+            __syncthreads();
         }
         if constexpr (!AM) {
             if (late) epi0(accp, thkp, nchunks - 1, bt, rs);
@@ -1086,7 +1063,7 @@ void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, f
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
 static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     // two query/threshold buffers, then (mode 0) the waves' candidate queues
-    const size_t lds = ((size_t)mfma_nbuf<NW, MAXS, NT, AM>() * mfma_buf_units<NW, MAXS, NT, AM>() +
+    const size_t lds = (2ull * (MAXS * 4 * 16 * NT + 8 * NT) +
                         (AM ? 0 : NW * kCandQueue * 12 / 16)) * sizeof(uint4);
     auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM>;
     // once per instantiation (also keeps the call out of graph captures)
