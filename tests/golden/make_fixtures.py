@@ -16,7 +16,7 @@ the expected FP64 top-5 of the 50,000 x 128 case of
 notebooks/python_bindings_demo.ipynb:33-37 (its inputs are regenerated from
 numpy's default_rng(42) at test time, so only the results are stored).
 
-Run from the repository root: python tests/golden/make_fixtures.py
+Run from the repository root: python tests/golden/make_fixtures.py [out.npz]
 """
 
 from __future__ import annotations
@@ -71,7 +71,7 @@ def cases():
     return out
 
 
-def main() -> None:
+def main(out: str = OUT) -> None:
     arrays = {}
     for name, (matrix, queries, k) in cases().items():
         rows, dims = matrix.shape
@@ -102,9 +102,9 @@ def main() -> None:
     g_idx, g_sim = fp64_topk(matrix, np.vstack([query[None, :], queries]), 5)
     arrays["golden50k/fp64_idx"] = g_idx
     arrays["golden50k/fp64_sim"] = g_sim
-    np.savez_compressed(OUT, **arrays)
-    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    np.savez_compressed(out, **arrays)
+    print("wrote", out, os.path.getsize(out), "bytes")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else OUT)

@@ -553,3 +553,22 @@ def test_oracle_reproduces_fixtures_and_agrees_with_fp64(orc, name):
         assert (f_sim < 0).all()
     if name == "ties":
         assert any(fx["ties/single_idx"][i].tolist() != f_idx[i].tolist() for i in range(6))
+
+
+def test_fixture_script_reproduces_committed_fixtures(tmp_path):
+    """The committed fixtures are what tests/golden/make_fixtures.py writes."""
+    import os
+    import subprocess
+    import sys
+
+    here = os.path.dirname(__file__)
+    out = str(tmp_path / "fx.npz")
+    subprocess.run([sys.executable, os.path.join(here, "golden", "make_fixtures.py"), out],
+                   check=True, capture_output=True)
+    with np.load(out) as a, np.load(os.path.join(here, "golden", "fixtures.npz")) as b:
+        assert sorted(a.files) == sorted(b.files)
+        for key in a.files:
+            if key.endswith("fp64_sim"):  # float64 products may differ in the last bit
+                np.testing.assert_allclose(a[key], b[key], rtol=0, atol=1e-12)
+            else:
+                assert np.array_equal(a[key], b[key]), key
