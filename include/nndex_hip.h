@@ -150,6 +150,23 @@ nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspa
                                            size_t k, uint64_t *device_indices,
                                            float *device_similarities, void *stream);
 
+/* Exact search that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as nndex_gpu_index_search_device (bit-identical
+ * outputs), without a workspace: every query runs the fused scan (up to 4 per
+ * launch), whose scratch is a static device array of 8 MiB of block-list keys
+ * plus 1 KiB of counters per GPU, held until the process ends.  Rejects with
+ * NNDEX_ERR_UNSUPPORTED a k whose block lists need more than that (scan grid
+ * x keys per query above 1,048,576; k above 1984).  Counters are cleared by a
+ * kernel at the start of each call.  Calls that overlap in time on one GPU
+ * share the array and race: serialise them (one stream, or events).  Never
+ * synchronises; no hipMalloc, hipFree or allocating library call. */
+nndex_status nndex_gpu_index_search_device_noalloc(nndex_gpu_index *index,
+                                                   const float *device_queries, size_t query_rows,
+                                                   size_t k, uint64_t *device_indices,
+                                                   float *device_similarities, void *stream);
+
 /* Candidate-scoring precision of the batched path.  FP8 is OCP E4M3 with a
  * per-row scale max|x_r| / 448 (and a per-query scale likewise). */
 typedef enum nndex_precision {

@@ -166,6 +166,7 @@ SIGNATURES = {
     "nndex_workspace_new": (_ST, [_P, _SZ, _SZ, ctypes.POINTER(_P)]),
     "nndex_workspace_free": (None, [_P]),
     "nndex_gpu_index_search_device": (_ST, [_P, _P, _P, _SZ, _SZ, _P, _P, _P]),
+    "nndex_gpu_index_search_device_noalloc": (_ST, [_P, _P, _SZ, _SZ, _P, _P, _P]),
     "nndex_gpu_index_search_batch_device": (_ST, [_P, _P, _P, _SZ, _SZ, _I32, _SZ, _P, _P, _P,
                                                   _P]),
     "nndex_gpu_index_autotune": (_ST, [_P, _I32, ctypes.c_char_p, _SZ]),

@@ -756,24 +756,57 @@ std::unique_ptr<Workspace> GpuIndex::make_workspace(size_t max_queries, size_t m
     return ws;
 }
 
-void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
-                             float *d_sim, hipStream_t s, bool heap_ties) const {
-    if (k == 0 || k > rows_)
-        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
-    if (k > ws.max_k)
-        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k exceeds the workspace's max_k");
-    // The reference heap's choice among rows tied at the k-th score is
-    // recovered from the canonical top 2k-1 (k above ~990: canonical order).
+// Keys the fused scan selects per query for k results: the reference heap's
+// choice among rows tied at the k-th score is recovered from the canonical
+// top 2k-1 (k above ~990: canonical order).
+static size_t scan_select_k(size_t k, size_t rows, bool heap_ties) {
     size_t ksel = k;
-    if (heap_ties && k > 1 && k < rows_) {
-        const size_t k2 = 2 * k - 1 < rows_ ? 2 * k - 1 : rows_;
+    if (heap_ties && k > 1 && k < rows) {
+        const size_t k2 = 2 * k - 1 < rows ? 2 * k - 1 : rows;
         if (scan_max_qb(static_cast<uint32_t>(k2)) > 0) ksel = k2;
     }
+    return ksel;
+}
+
+size_t GpuIndex::scan_part_keys(size_t k, bool heap_ties) const {
+    return static_cast<size_t>(scan_grid()) * scan_select_k(k, rows_, heap_ties);
+}
+
+void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
+                             float *d_sim, hipStream_t s, bool heap_ties) const {
+    if (k > ws.max_k && k <= rows_)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k exceeds the workspace's max_k");
+    if (static_cast<size_t>(scan_grid()) * 2048 > ws.part_elems)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT,
+                    "workspace was created for a smaller launch shape; create a new one");
+    ScanScratch sc;
+    sc.part = ws.part;
+    sc.part_elems = ws.part_elems;
+    sc.part_count = ws.part_count;
+    sc.theta = ws.theta;
+    sc.ticket = ws.ticket;
+    search_device(sc, dq, nq, k, d_idx, d_sim, s, heap_ties);
+}
+
+void GpuIndex::search_device(const ScanScratch &sc, const float *dq, size_t nq, size_t k,
+                             uint64_t *d_idx, float *d_sim, hipStream_t s, bool heap_ties) const {
+    if (k == 0 || k > rows_)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+    const size_t ksel = scan_select_k(k, rows_, heap_ties);
     if (scan_max_qb(static_cast<uint32_t>(k)) == 0)
         throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
-    const uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(ksel));
+    uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(ksel));
     if (qbmax == 0)
         throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
+    // every launch publishes at most grid x ksel keys per query; take as many
+    // queries per launch as the scratch holds (results do not depend on it)
+    const size_t per_query = static_cast<size_t>(scan_grid()) * ksel;
+    while (qbmax > 1 && qbmax * per_query > sc.part_elems) qbmax >>= 1;
+    if (per_query > sc.part_elems)
+        throw Error(NNDEX_ERR_UNSUPPORTED,
+                    "k=" + std::to_string(k) + " needs " + std::to_string(per_query) +
+                        " block-list keys per launch, above the scratch's " +
+                        std::to_string(sc.part_elems));
     for (size_t q0 = 0; q0 < nq; q0 += qbmax) {
         uint32_t nvalid = static_cast<uint32_t>(nq - q0 < qbmax ? nq - q0 : qbmax);
         uint32_t qb = 1;
@@ -787,18 +820,17 @@ void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k
         a.nvalid = nvalid;
         a.k = static_cast<uint32_t>(ksel);
         a.k_out = static_cast<uint32_t>(k);
-        a.part = ws.part;
-        a.part_count = ws.part_count;
-        a.theta = ws.theta;
-        a.ticket = ws.ticket;
+        a.part = sc.part;
+        a.part_count = sc.part_count;
+        a.theta = sc.theta;
+        a.ticket = sc.ticket;
         a.out_idx = d_idx + q0 * k;
         a.out_sim = d_sim + q0 * k;
         a.out_stride = static_cast<uint32_t>(k);
         a.flags = scan_flags_;
         const uint32_t grid = scan_grid();
-        if (static_cast<size_t>(grid) * 2048 > ws.part_elems)
-            throw Error(NNDEX_ERR_INVALID_ARGUMENT,
-                        "workspace was created for a smaller launch shape; create a new one");
+        if (static_cast<size_t>(qb) * grid * ksel > sc.part_elems)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "fused scan block lists exceed the scratch");
         launch_exact_scan(a, qb, grid, s);
     }
 }
@@ -1273,6 +1305,87 @@ nndex_status nndex_gpu_index_search_batch_device(nndex_gpu_index *index, nndex_w
                                        static_cast<Precision>(precision), overfetch,
                                        device_indices, device_similarities, device_flags,
                                        static_cast<hipStream_t>(stream));
+    });
+}
+
+}  // extern "C"
+
+// ---- Exact search without allocation ---------------------------------------
+// The fused scan's scratch lives in this file-scope device array instead of a
+// workspace: the block lists (kNoAllocPartKeys 64-bit keys, 8 MiB), then at
+// kNoAllocCounters the live counts [8], the thresholds [8] and the ticket,
+// each 256-byte aligned.  Every device the library runs on holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in:
+// the device address comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr size_t kNoAllocPartKeys = size_t(1) << 20;
+constexpr size_t kNoAllocCounters = kNoAllocPartKeys * sizeof(uint64_t);
+constexpr size_t kNoAllocBytes = kNoAllocCounters + 1024;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on `device` (looked up on the first call there,
+// with that device current; the caller's current device is restored).
+static unsigned char *scratch_on_device(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "device index beyond the no-allocation table");
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        NNDEX_HIP_CHECK(hipGetDevice(&cur));
+        if (cur != device) NNDEX_HIP_CHECK(hipSetDevice(device));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess)
+            throw Error(NNDEX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                           hipGetErrorString(e));
+        if (!ptr) throw Error(NNDEX_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[device] = ptr;
+    }
+    return static_cast<unsigned char *>(g_scratch_ptr[device]);
+}
+
+extern "C" {
+
+nndex_status nndex_gpu_index_search_device_noalloc(nndex_gpu_index *index,
+                                                   const float *device_queries, size_t query_rows,
+                                                   size_t k, uint64_t *device_indices,
+                                                   float *device_similarities, void *stream) {
+    return guard([&] {
+        if (!index || !device_indices || !device_similarities ||
+            (!device_queries && query_rows))
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
+        const GpuIndex &g = *index->impl;
+        if (k > g.rows()) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+        if (scan_max_qb(static_cast<uint32_t>(k)) == 0)
+            throw Error(NNDEX_ERR_UNSUPPORTED,
+                        "k=" + std::to_string(k) + " exceeds the fused scan (k <= 1984)");
+        const size_t need = g.scan_part_keys(k);
+        if (need > kNoAllocPartKeys)
+            throw Error(NNDEX_ERR_UNSUPPORTED,
+                        "k=" + std::to_string(k) + " needs " + std::to_string(need) +
+                            " block-list keys, above the no-allocation scratch's " +
+                            std::to_string(kNoAllocPartKeys) + " (8 MiB)");
+        if (query_rows == 0) return;
+        unsigned char *base = scratch_on_device(g.device());
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        ScanScratch sc;
+        sc.part = reinterpret_cast<uint64_t *>(base);
+        sc.part_elems = kNoAllocPartKeys;
+        sc.part_count = reinterpret_cast<uint32_t *>(base + kNoAllocCounters);
+        sc.theta = reinterpret_cast<unsigned long long *>(base + kNoAllocCounters + 256);
+        sc.ticket = reinterpret_cast<uint32_t *>(base + kNoAllocCounters + 512);
+        // counts, thresholds and ticket start at zero (a kernel, not a
+        // memset node, so the call may be captured into a graph)
+        launch_zero_u32(reinterpret_cast<uint32_t *>(base + kNoAllocCounters), 1024 / 4, s);
+        g.search_device(sc, device_queries, query_rows, k, device_indices, device_similarities,
+                        s);
     });
 }
 

@@ -74,6 +74,19 @@ struct Workspace {
     ~Workspace();
 };
 
+// Scratch of one fused-scan launch: the blocks' published keys, the live
+// counts, the (unused) device-wide thresholds and the block ticket; counts,
+// thresholds and ticket are zero between launches (the last block resets
+// them).  A Workspace provides it, or the no-allocation entry point's static
+// device array.
+struct ScanScratch {
+    uint64_t *part = nullptr;
+    size_t part_elems = 0;
+    uint32_t *part_count = nullptr;   // [8]
+    unsigned long long *theta = nullptr;  // [8]
+    uint32_t *ticket = nullptr;
+};
+
 // Sizes of the batched MFMA path for one (k, precision): over-fetch k',
 // sample stride and rows, candidate capacity per query, queries per group.
 // Exact batches of up to this many queries run the fused scan (one launch);
@@ -171,6 +184,13 @@ class GpuIndex {
     // desc, row asc) order.
     void search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
                        float *d_sim, hipStream_t s, bool heap_ties = true) const;
+    // The same fused scan over caller-provided scratch: each launch takes as
+    // many queries (at most 4) as the scratch holds block lists for; throws
+    // UNSUPPORTED when not even one query's lists fit.
+    void search_device(const ScanScratch &sc, const float *dq, size_t nq, size_t k,
+                       uint64_t *d_idx, float *d_sim, hipStream_t s, bool heap_ties = true) const;
+    // Block-list keys one single-query fused-scan launch of k results needs.
+    size_t scan_part_keys(size_t k, bool heap_ties = true) const;
 
     // Enqueue the batched MFMA search with exact rerank and certificate
     // (no host synchronisation).  flags[q] = 1 marks a query whose result is
