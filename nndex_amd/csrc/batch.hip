@@ -22,6 +22,8 @@
 // whole batch and queries stream through LDS in chunks of 16, so the index is
 // read from HBM once per batch whatever the batch size.
 #include "batch.hpp"
+
+#include <mutex>
 #include "common.hpp"
 #include "wave_topk.hpp"
 
@@ -1066,13 +1068,16 @@ static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     const size_t lds = (2ull * (MAXS * 4 * 16 * NT + 8 * NT) +
                         (AM ? 0 : NW * kCandQueue * 12 / 16)) * sizeof(uint4);
     auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM>;
-    // once per instantiation (also keeps the call out of graph captures)
-    static bool attr_set = false;
-    if (!attr_set) {
-        NNDEX_HIP_CHECK(hipFuncSetAttribute((const void *)fn,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        attr_set = true;
-    }
+    // once per instantiation (also keeps the call out of graph captures);
+    // serving executors launch from several threads, so the first call sets
+    // it and every other waits until it is set
+    static std::once_flag attr_once;
+    static hipError_t attr_err = hipSuccess;
+    std::call_once(attr_once, [&] {
+        attr_err = hipFuncSetAttribute((const void *)fn,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    });
+    NNDEX_HIP_CHECK(attr_err);
     hipLaunchKernelGGL(fn, dim3(grid), dim3(NW * 64), lds, s, a);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
