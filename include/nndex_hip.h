@@ -158,7 +158,9 @@ nndex_status nndex_gpu_index_search_device(nndex_gpu_index *index, nndex_workspa
  * launch), whose scratch is a static device array of 8 MiB of block-list keys
  * plus 1 KiB of counters per GPU, held until the process ends.  Rejects with
  * NNDEX_ERR_UNSUPPORTED a k whose block lists need more than that (scan grid
- * x keys per query above 1,048,576; k above 1984).  Counters are cleared by a
+ * x keys per query above 1,048,576) and a k the fused scan cannot hold
+ * (above 992; above 1984 when k equals rows), which the workspace entry point
+ * serves with its large-k selection.  Counters are cleared by a
  * kernel at the start of each call.  Calls that overlap in time on one GPU
  * share the array and race: serialise them (one stream, or events).  Never
  * synchronises; no hipMalloc, hipFree or allocating library call. */

@@ -44,19 +44,75 @@ PinnedBuffer::~PinnedBuffer() {
     if (ptr) (void)hipHostFree(ptr);
 }
 
-Workspace::~Workspace() {
-    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta, (void *)qscale,
-                    (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
-                    (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
-                    (void *)xflags, (void *)ivf_pair_u32, (void *)ivf_list_u32, ivf_sort_tmp})
-        if (p) (void)hipFree(p);
-}
-
 template <typename T>
 static void regrow(T *&p, size_t n) {
     if (p) NNDEX_HIP_CHECK(hipFree(p));
     p = nullptr;
     NNDEX_HIP_CHECK(hipMalloc(&p, n * sizeof(T) + 256));
+}
+
+Workspace::~Workspace() {
+    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta, (void *)qscale,
+                    (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
+                    (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
+                    (void *)xflags, (void *)ivf_pair_u32, (void *)ivf_list_u32, ivf_sort_tmp,
+                    (void *)sel_keys, (void *)sel_cand, (void *)sel_sorted, (void *)sel_st,
+                    (void *)sel_hist, (void *)sel_cnt, (void *)sel_offs, sel_tmp})
+        if (p) (void)hipFree(p);
+}
+
+void Workspace::reserve_select(size_t nq, size_t ld, size_t ksel) {
+    const bool grow_q = nq > sel_nq, grow_ld = grow_q || ld > sel_ld,
+               grow_k = grow_q || ksel > sel_k;
+    if (!grow_q && !grow_ld && !grow_k) return;
+    ++generation;
+    if (grow_ld) {
+        regrow(sel_keys, nq * ld);
+        sel_ld = ld;
+    }
+    if (grow_k) {
+        regrow(sel_cand, nq * ksel);
+        regrow(sel_sorted, nq * ksel);
+        sel_k = ksel;
+    }
+    if (grow_q) {
+        regrow(sel_st, nq);
+        regrow(sel_hist, nq * kSelectBins);
+        regrow(sel_cnt, nq);
+        regrow(sel_offs, 2 * nq);
+        sel_nq = nq;
+        // histograms start at zero (the selection keeps them zero after)
+        NNDEX_HIP_CHECK(hipMemset(sel_hist, 0, nq * kSelectBins * sizeof(uint32_t)));
+    }
+    const size_t bytes = select_sort_bytes(sel_nq, sel_k);
+    if (bytes > sel_tmp_bytes) {
+        if (sel_tmp) NNDEX_HIP_CHECK(hipFree(sel_tmp));
+        sel_tmp = nullptr;
+        NNDEX_HIP_CHECK(hipMalloc(&sel_tmp, bytes));
+        sel_tmp_bytes = bytes;
+    }
+}
+
+SelectBuffers Workspace::select_buffers(uint32_t nq, uint32_t ksel, const uint32_t *n,
+                                        uint32_t n_all) const {
+    if (nq > sel_nq || ksel > sel_k)
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "selection buffers are smaller than the launch");
+    SelectBuffers b;
+    b.keys = sel_keys;
+    b.ld = sel_ld;
+    b.n = n;
+    b.n_all = n_all;
+    b.nq = nq;
+    b.ksel = ksel;
+    b.st = sel_st;
+    b.hist = sel_hist;
+    b.cand = sel_cand;
+    b.sorted = sel_sorted;
+    b.cnt = sel_cnt;
+    b.offs = sel_offs;
+    b.sort_tmp = sel_tmp;
+    b.sort_bytes = sel_tmp_bytes;
+    return b;
 }
 
 void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t capq) {
@@ -472,6 +528,14 @@ void Workspace::reserve_xflags(size_t nq) {
 void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hipStream_t s) {
     NNDEX_HIP_CHECK(hipSetDevice(device_));
     if (k > ws.max_k) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k exceeds the workspace's max_k");
+    if (k > rows_) k = rows_;
+    // large k: the selection buffers for the largest k (each k up to it
+    // selects fewer keys)
+    for (size_t kk : {k, k - 1})
+        if (kk >= 1 && !fused_scan_serves(kk)) {
+            ws.reserve_select(4, rows_, 2 * kk - 1 < rows_ ? 2 * kk - 1 : rows_);
+            break;
+        }
     Precision p;
     if (nq > kFusedBatchMax && exact_batch_precision(p)) {
         // every k up to the maximum: the sample stride and the candidate
@@ -517,6 +581,11 @@ void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hi
 
 size_t GpuIndex::workspace_bytes(size_t nq, size_t k, size_t ivf_nprobe) {
     size_t bytes = static_cast<size_t>(ncu_) * 8 * (2048 * 8 + 8 * 4) + 256;
+    if (k > rows_) k = rows_;
+    if (k >= 1 && (!fused_scan_serves(k) || (k > 1 && !fused_scan_serves(k - 1)))) {
+        const size_t ksel = 2 * k - 1 < rows_ ? 2 * k - 1 : rows_;
+        bytes += 4 * (rows_ + 2 * ksel) * 8 + 4 * (kSelectBins + 8) * 4 + select_sort_bytes(4, ksel);
+    }
     Precision p;
     if (nq > kFusedBatchMax && exact_batch_precision(p)) {
         size_t most = 0;
@@ -772,6 +841,34 @@ size_t GpuIndex::scan_part_keys(size_t k, bool heap_ties) const {
     return static_cast<size_t>(scan_grid()) * scan_select_k(k, rows_, heap_ties);
 }
 
+bool GpuIndex::fused_scan_serves(size_t k, bool heap_ties) const {
+    if (heap_ties && k > 1 && k < rows_) {
+        const size_t k2 = 2 * k - 1 < rows_ ? 2 * k - 1 : rows_;
+        return scan_max_qb(static_cast<uint32_t>(k2)) > 0;
+    }
+    return scan_max_qb(static_cast<uint32_t>(k)) > 0;
+}
+
+void GpuIndex::search_large_k(Workspace &ws, const float *dq, size_t nq, size_t k,
+                              uint64_t *d_idx, float *d_sim, hipStream_t s, bool heap_ties) const {
+    if (k == 0 || k > rows_) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
+    // the canonical top 2k - 1 holds every row the heap's tie rule can need
+    const size_t ksel = heap_ties && k > 1 && k < rows_ ? (2 * k - 1 < rows_ ? 2 * k - 1 : rows_)
+                                                        : k;
+    constexpr size_t group = 4;
+    ws.reserve_select(group, rows_, ksel);
+    for (size_t q0 = 0; q0 < nq; q0 += group) {
+        const uint32_t n = static_cast<uint32_t>(nq - q0 < group ? nq - q0 : group);
+        launch_score_keys(x_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(d4_),
+                          dq + q0 * pdims_, n, ws.sel_keys, ws.sel_ld, ncu_, s);
+        const SelectBuffers b = ws.select_buffers(n, static_cast<uint32_t>(ksel), nullptr,
+                                                  static_cast<uint32_t>(rows_));
+        select_top(b, s);
+        launch_finish_exact(b, static_cast<uint32_t>(k), heap_ties, d_idx + q0 * k,
+                            d_sim + q0 * k, k, s);
+    }
+}
+
 void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
                              float *d_sim, hipStream_t s, bool heap_ties) const {
     if (k > ws.max_k && k <= rows_)
@@ -779,6 +876,10 @@ void GpuIndex::search_device(Workspace &ws, const float *dq, size_t nq, size_t k
     if (static_cast<size_t>(scan_grid()) * 2048 > ws.part_elems)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT,
                     "workspace was created for a smaller launch shape; create a new one");
+    if (k >= 1 && k <= rows_ && !fused_scan_serves(k, heap_ties)) {
+        search_large_k(ws, dq, nq, k, d_idx, d_sim, s, heap_ties);
+        return;
+    }
     ScanScratch sc;
     sc.part = ws.part;
     sc.part_elems = ws.part_elems;
@@ -793,8 +894,11 @@ void GpuIndex::search_device(const ScanScratch &sc, const float *dq, size_t nq, 
     if (k == 0 || k > rows_)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
     const size_t ksel = scan_select_k(k, rows_, heap_ties);
-    if (scan_max_qb(static_cast<uint32_t>(k)) == 0)
-        throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
+    if (!fused_scan_serves(k, heap_ties))
+        throw Error(NNDEX_ERR_UNSUPPORTED,
+                    "k=" + std::to_string(k) +
+                        " exceeds the fused scan (k <= 992, or k = rows <= 1984); larger k "
+                        "needs a workspace");
     uint32_t qbmax = scan_max_qb(static_cast<uint32_t>(ksel));
     if (qbmax == 0)
         throw Error(NNDEX_ERR_UNSUPPORTED, "k=" + std::to_string(k) + " exceeds the fused scan");
@@ -1363,9 +1467,10 @@ nndex_status nndex_gpu_index_search_device_noalloc(nndex_gpu_index *index,
         if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
         const GpuIndex &g = *index->impl;
         if (k > g.rows()) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "k must be in [1, rows]");
-        if (scan_max_qb(static_cast<uint32_t>(k)) == 0)
+        if (!g.fused_scan_serves(k))
             throw Error(NNDEX_ERR_UNSUPPORTED,
-                        "k=" + std::to_string(k) + " exceeds the fused scan (k <= 1984)");
+                        "k=" + std::to_string(k) +
+                            " exceeds the fused scan (k <= 992, or k = rows <= 1984)");
         const size_t need = g.scan_part_keys(k);
         if (need > kNoAllocPartKeys)
             throw Error(NNDEX_ERR_UNSUPPORTED,

@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "ivf.hpp"
 #include "kernels.hpp"
+#include "select.hpp"
 
 namespace nndex {
 
@@ -71,6 +72,18 @@ struct Workspace {
     uint8_t *xflags = nullptr;
     size_t xflags_n = 0;
     void reserve_xflags(size_t nq);
+    // large-k selection (select.hip): per group of nq queries, every row's
+    // key [nq][ld], the selected keys [nq][ksel] before and after sorting,
+    // the radix-select state, histograms, counts, segment offsets and rocPRIM
+    // scratch
+    size_t sel_nq = 0, sel_ld = 0, sel_k = 0, sel_tmp_bytes = 0;
+    uint64_t *sel_keys = nullptr, *sel_cand = nullptr, *sel_sorted = nullptr;
+    SelState *sel_st = nullptr;
+    uint32_t *sel_hist = nullptr, *sel_cnt = nullptr, *sel_offs = nullptr;
+    void *sel_tmp = nullptr;
+    void reserve_select(size_t nq, size_t ld, size_t ksel);
+    SelectBuffers select_buffers(uint32_t nq, uint32_t ksel, const uint32_t *n,
+                                 uint32_t n_all) const;
     ~Workspace();
 };
 
@@ -191,6 +204,15 @@ class GpuIndex {
                        uint64_t *d_idx, float *d_sim, hipStream_t s, bool heap_ties = true) const;
     // Block-list keys one single-query fused-scan launch of k results needs.
     size_t scan_part_keys(size_t k, bool heap_ties = true) const;
+    // Whether the fused scan's LDS lists hold what k results need (with
+    // heap_ties, the canonical top 2k - 1); otherwise the exact search runs
+    // the large-k selection.
+    bool fused_scan_serves(size_t k, bool heap_ties = true) const;
+    // Exact search for any k: every row's key, radix select of the top
+    // min(2k - 1, rows), sort, the heap's tie rule (allocates its buffers in
+    // the workspace on first use).
+    void search_large_k(Workspace &ws, const float *dq, size_t nq, size_t k, uint64_t *d_idx,
+                        float *d_sim, hipStream_t s, bool heap_ties = true) const;
 
     // Enqueue the batched MFMA search with exact rerank and certificate
     // (no host synchronisation).  flags[q] = 1 marks a query whose result is

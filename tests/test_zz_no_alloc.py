@@ -73,8 +73,10 @@ def test_noalloc_rejects_what_it_cannot_hold(orc):
     dq = DevBuf.from_array(m[:2])
     di, dsim = DevBuf(2 * 2000 * 8), DevBuf(2 * 2000 * 4)
     assert _noalloc(b, dq, 2, 0, di, dsim, None) == _lib.INVALID_TOP_K
-    assert _noalloc(b, dq, 2, 1985, di, dsim, None) == _lib.UNSUPPORTED
+    # k above 992 needs the large-k selection's buffers (a workspace)
+    assert _noalloc(b, dq, 2, 993, di, dsim, None) == _lib.UNSUPPORTED
     assert b"fused scan" in _lib.lib().nndex_last_error()
+    assert _noalloc(b, dq, 2, 1985, di, dsim, None) == _lib.UNSUPPORTED
     assert _noalloc(b, dq, 2, 5001, di, dsim, None) == _lib.INVALID_ARGUMENT
     # zero queries: nothing to do
     assert _noalloc(b, dq, 0, 5, di, dsim, None) == _lib.OK
