@@ -57,33 +57,34 @@ Workspace::~Workspace() {
                     (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
                     (void *)xflags, (void *)ivf_pair_u32, (void *)ivf_list_u32, ivf_sort_tmp,
                     (void *)sel_keys, (void *)sel_cand, (void *)sel_sorted, (void *)sel_st,
-                    (void *)sel_hist, (void *)sel_cnt, (void *)sel_offs, sel_tmp})
+                    (void *)sel_hist, (void *)sel_cnt, (void *)sel_offs, (void *)sel_n,
+                    sel_tmp})
         if (p) (void)hipFree(p);
 }
 
 void Workspace::reserve_select(size_t nq, size_t ld, size_t ksel) {
-    const bool grow_q = nq > sel_nq, grow_ld = grow_q || ld > sel_ld,
-               grow_k = grow_q || ksel > sel_k;
-    if (!grow_q && !grow_ld && !grow_k) return;
+    // every buffer sized for the largest group, stride and k seen so far
+    // (the layout strides are sel_ld and sel_k whatever the call asks)
+    const size_t nn = nq > sel_nq ? nq : sel_nq, nld = ld > sel_ld ? ld : sel_ld,
+                 nk = ksel > sel_k ? ksel : sel_k;
+    if (nn == sel_nq && nld == sel_ld && nk == sel_k) return;
     ++generation;
-    if (grow_ld) {
-        regrow(sel_keys, nq * ld);
-        sel_ld = ld;
+    if (nn != sel_nq || nld != sel_ld) regrow(sel_keys, nn * nld);
+    if (nn != sel_nq || nk != sel_k) {
+        regrow(sel_cand, nn * nk);
+        regrow(sel_sorted, nn * nk);
     }
-    if (grow_k) {
-        regrow(sel_cand, nq * ksel);
-        regrow(sel_sorted, nq * ksel);
-        sel_k = ksel;
+    if (nn != sel_nq) {
+        // histograms are cleared by each selection's first kernel
+        regrow(sel_st, nn);
+        regrow(sel_hist, nn * kSelectBins);
+        regrow(sel_cnt, nn);
+        regrow(sel_offs, 2 * nn);
+        regrow(sel_n, nn);
     }
-    if (grow_q) {
-        regrow(sel_st, nq);
-        regrow(sel_hist, nq * kSelectBins);
-        regrow(sel_cnt, nq);
-        regrow(sel_offs, 2 * nq);
-        sel_nq = nq;
-        // histograms start at zero (the selection keeps them zero after)
-        NNDEX_HIP_CHECK(hipMemset(sel_hist, 0, nq * kSelectBins * sizeof(uint32_t)));
-    }
+    sel_nq = nn;
+    sel_ld = nld;
+    sel_k = nk;
     const size_t bytes = select_sort_bytes(sel_nq, sel_k);
     if (bytes > sel_tmp_bytes) {
         if (sel_tmp) NNDEX_HIP_CHECK(hipFree(sel_tmp));
@@ -571,6 +572,17 @@ void GpuIndex::prepare(Workspace &ws, size_t nq, size_t k, size_t ivf_nprobe, hi
                         part = e > part ? e : part;
                     }
             ws.reserve_ivf(nq * np, ivf.nlist, part, ivf_sort_bytes(nq * np, ivf.nlist));
+            if (ivf_large_k(k)) {
+                // the largest group (fewest probes) and the longest key arrays
+                size_t gmax = 0, ldmax = 0;
+                for (size_t p = 1; p <= np; ++p) {
+                    size_t g = 0, ld = 0;
+                    ivf_large_k_shape(ivf, nq, p, k, g, ld);
+                    gmax = g > gmax ? g : gmax;
+                    ldmax = ld > ldmax ? ld : ldmax;
+                }
+                ws.reserve_select(gmax, ldmax, 2 * k - 1);
+            }
             if (!ws.sub || ws.sub->max_k < np)
                 ws.sub = ivf.cindex->make_workspace(8, np < 10 ? 10 : np);
             ivf.cindex->prepare(*ws.sub, nq, np, 0, s);
@@ -600,6 +612,14 @@ size_t GpuIndex::workspace_bytes(size_t nq, size_t k, size_t ivf_nprobe) {
         if (ivf.nlist >= 2) {
             const size_t np = ivf_nprobe < ivf.nlist ? ivf_nprobe : ivf.nlist;
             bytes += nq * np * (8 + 4) + ivf_scan_scratch_bytes(nq, np, ivf.nlist, rows_, k);
+            if (ivf_large_k(k)) {
+                size_t g = 0, ld = 0;
+                ivf_large_k_shape(ivf, nq, 1, k, g, ld);
+                size_t g2 = 0, ld2 = 0;
+                ivf_large_k_shape(ivf, nq, np, k, g2, ld2);
+                bytes += g * ld2 * 8 + g * (2 * (2 * k - 1) * 8 + (kSelectBins + 16) * 4) +
+                         select_sort_bytes(g, 2 * k - 1);
+            }
             bytes += ivf.cindex->workspace_bytes(nq, np, 0);
         }
     }
@@ -1211,17 +1231,65 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     if (nprobe == 0) nprobe = ivf_nprobe_;
     if (nprobe == 0) nprobe = ivf_choose_nprobe(ivf.nlist, k, rows_);
     if (nprobe > ivf.nlist) nprobe = ivf.nlist;
+    if (!ws.sub || ws.sub->max_k < nprobe)
+        ws.sub = ivf.cindex->make_workspace(8, nprobe < 10 ? 10 : nprobe);
+    if (ivf_large_k(k)) {
+        // every probed row's key, then the top 2k - 1 per query by radix
+        // select, in groups of queries whose key arrays fit 256 MiB
+        size_t group = 0, ld = 0;
+        ivf_large_k_shape(ivf, nq, nprobe, k, group, ld);
+        const size_t ksel = 2 * k - 1;
+        ws.reserve_select(group, ld, ksel);
+        for (size_t q0 = 0; q0 < nq; q0 += group) {
+            const size_t n = nq - q0 < group ? nq - q0 : group;
+            uint32_t qb, nchunk;
+            ivf_scan_shape(n, nprobe, ivf.nlist, rows_, 1, qb, nchunk);
+            const size_t pairs = n * nprobe;
+            ws.reserve_ivf(pairs, ivf.nlist, 0, ivf_sort_bytes(pairs, ivf.nlist));
+            ivf.cindex->search_exact_any(*ws.sub, dq + q0 * pdims_, n, nprobe, ws.ivf_probe,
+                                         ws.ivf_probe_sim, s, false);
+            IvfScanArgs a = ivf_scan_args(ivf, ws, dq + q0 * pdims_, n, k, nprobe, 1, qb, nchunk,
+                                          d_idx + q0 * k, d_sim + q0 * k, d_len + q0);
+            a.keys_out = ws.sel_keys;
+            a.keys_ld = ws.sel_ld;
+            a.totals = ws.sel_n;
+            ivf_scan(a, s);
+            const SelectBuffers b =
+                ws.select_buffers(static_cast<uint32_t>(n), static_cast<uint32_t>(ksel), ws.sel_n, 0);
+            select_top(b, s);
+            ivf_finish_large_k(a, b, static_cast<uint32_t>(k), s);
+        }
+        return;
+    }
     uint32_t qb, nchunk;
     const uint32_t kk = ivf_select_k(k);
     ivf_scan_shape(nq, nprobe, ivf.nlist, rows_, kk, qb, nchunk);
     const size_t pairs = nq * nprobe;
     ws.reserve_ivf(pairs, ivf.nlist, pairs * nchunk * kk, ivf_sort_bytes(pairs, ivf.nlist));
-    if (!ws.sub || ws.sub->max_k < nprobe)
-        ws.sub = ivf.cindex->make_workspace(8, nprobe < 10 ? 10 : nprobe);
     // candidate_indices (approx.rs:110-138): top-nprobe centroids by score
     // the probe ranks centroids in (similarity desc, index asc) order
     ivf.cindex->search_exact_any(*ws.sub, dq, nq, nprobe, ws.ivf_probe, ws.ivf_probe_sim, s,
                                  false);
+    ivf_scan(ivf_scan_args(ivf, ws, dq, nq, k, nprobe, kk, qb, nchunk, d_idx, d_sim, d_len), s);
+}
+
+void GpuIndex::ivf_large_k_shape(const IvfIndex &ivf, size_t nq, size_t nprobe, size_t k,
+                                 size_t &group, size_t &ld) const {
+    ld = ivf.max_probe_rows(nprobe);
+    if (ld < 1) ld = 1;
+    group = (size_t(256) << 20) / (ld * sizeof(uint64_t));
+    const size_t by_k = (size_t(256) << 20) / ((2 * k - 1) * 2 * sizeof(uint64_t));
+    if (group > by_k) group = by_k;
+    if (group < 1) group = 1;
+    if (group > 64) group = 64;
+    if (group > nq) group = nq;
+}
+
+IvfScanArgs GpuIndex::ivf_scan_args(const IvfIndex &ivf, Workspace &ws, const float *dq,
+                                    size_t nq, size_t k, size_t nprobe, uint32_t kk, uint32_t qb,
+                                    uint32_t nchunk, uint64_t *d_idx, float *d_sim,
+                                    uint64_t *d_len) const {
+    const size_t pairs = nq * nprobe;
     IvfScanArgs a{};
     a.x = ivf.xr;
     a.d4 = static_cast<uint32_t>(d4_);
@@ -1252,7 +1320,7 @@ void GpuIndex::search_ivf_device(Workspace &ws, const float *dq, size_t nq, size
     a.out_idx = d_idx;
     a.out_sim = d_sim;
     a.out_len = d_len;
-    ivf_scan(a, s);
+    return a;
 }
 
 }  // namespace nndex

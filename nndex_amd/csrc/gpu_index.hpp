@@ -80,6 +80,7 @@ struct Workspace {
     uint64_t *sel_keys = nullptr, *sel_cand = nullptr, *sel_sorted = nullptr;
     SelState *sel_st = nullptr;
     uint32_t *sel_hist = nullptr, *sel_cnt = nullptr, *sel_offs = nullptr;
+    uint32_t *sel_n = nullptr;  // [nq] candidates per query (IVF: probed rows)
     void *sel_tmp = nullptr;
     void reserve_select(size_t nq, size_t ld, size_t ksel);
     SelectBuffers select_buffers(uint32_t nq, uint32_t ksel, const uint32_t *n,
@@ -264,6 +265,14 @@ class GpuIndex {
     // needed); lens[q] = results found.
     void search_ivf_device(Workspace &ws, const float *dq, size_t nq, size_t k, size_t nprobe,
                            uint64_t *d_idx, float *d_sim, uint64_t *d_len, hipStream_t s);
+    // Scan arguments of one IVF search over the workspace's buffers.
+    IvfScanArgs ivf_scan_args(const IvfIndex &ivf, Workspace &ws, const float *dq, size_t nq,
+                              size_t k, size_t nprobe, uint32_t kk, uint32_t qb, uint32_t nchunk,
+                              uint64_t *d_idx, float *d_sim, uint64_t *d_len) const;
+    // Queries per group of the IVF large-k form (their key arrays and their
+    // selected keys within 256 MiB each) and the key array length per query.
+    void ivf_large_k_shape(const IvfIndex &ivf, size_t nq, size_t nprobe, size_t k,
+                           size_t &group, size_t &ld) const;
     // An index over rows already on the device (row-major, padded dims).
     static std::unique_ptr<GpuIndex> from_device_rows(const float *d_rows, size_t rows,
                                                       size_t padded_dims, int device);

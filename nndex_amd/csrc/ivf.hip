@@ -31,6 +31,7 @@
 // FP32 value the reference computes.
 #include <cstring>  // rocPRIM's headers use memset
 
+#include <algorithm>
 #include <chrono>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <vector>
@@ -39,6 +40,7 @@
 #include "common.hpp"
 #include "gpu_index.hpp"
 #include "ivf.hpp"
+#include "select.hpp"
 #include "wave_topk.hpp"
 
 namespace nndex {
@@ -537,7 +539,7 @@ __global__ void pair_prep_kernel(const uint64_t *__restrict__ probe, uint32_t pa
 // before it (the reference pushes rows list by list in probe order).
 __global__ void pair_base_kernel(const uint64_t *__restrict__ probe, uint32_t nq, uint32_t nprobe,
                                  const uint32_t *__restrict__ offsets,
-                                 uint32_t *__restrict__ base) {
+                                 uint32_t *__restrict__ base, uint32_t *__restrict__ totals) {
     for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < nq; q += gridDim.x * 256) {
         uint32_t acc = 0;
         for (uint32_t p = 0; p < nprobe; ++p) {
@@ -545,6 +547,7 @@ __global__ void pair_base_kernel(const uint64_t *__restrict__ probe, uint32_t nq
             const uint32_t c = (uint32_t)probe[(size_t)q * nprobe + p];
             acc += offsets[c + 1] - offsets[c];
         }
+        if (totals) totals[q] = acc;
     }
 }
 
@@ -562,9 +565,11 @@ __global__ void group_count_kernel(const uint32_t *__restrict__ pair_cnt, uint32
 // writes its best kk keys for this chunk.  A key's row part is the row's
 // arrival index in the query's probe sequence (list by list in probe order,
 // reordered position within a list), the order the reference pushes rows in.
-template <int QB, int CW>
+// KEYS (the large-k form): no top-k; every row's key goes to
+// keys_out[query][arrival].
+template <int QB, int CW, bool KEYS = false>
 __global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
-    __shared__ uint64_t slots[4][QB][CW];
+    __shared__ uint64_t slots[4][QB][KEYS ? 1 : CW];
     // wave-uniform by construction; readfirstlane lets the compiler keep the
     // work item, the list and the query pointers in scalar registers
     const uint32_t lane = __lane_id();
@@ -597,8 +602,10 @@ __global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
     const uint32_t r0 = l0 + (chunk * per < len ? chunk * per : len);
     const uint32_t r1 = l0 + ((chunk + 1) * per < len ? (chunk + 1) * per : len);
     WaveTopK tk[QB];
+    if constexpr (!KEYS) {
 #pragma unroll
-    for (int j = 0; j < QB; ++j) tk[j].init(slots[wid][j], CW, a.kk, nullptr, a.k);
+        for (int j = 0; j < QB; ++j) tk[j].init(slots[wid][j], CW, a.kk, nullptr, a.k);
+    }
     uint32_t abase[QB];
 #pragma unroll
     for (int j = 0; j < QB; ++j) abase[j] = a.pair_base[pid[j]] - l0;
@@ -625,9 +632,19 @@ __global__ __launch_bounds__(256) void ivf_list_scan_kernel(IvfScanArgs a) {
         for (int j = 0; j < QB; ++j) {
             if ((uint32_t)j >= pn) break;
             const float sc = (acc[j][0] + acc[j][1]) + (acc[j][2] + acc[j][3]);
-            tk[j].offer(valid ? make_key(sc, abase[j] + row) : 0, valid);
+            if constexpr (KEYS) {
+                // the arrival index in 32 bits: abase wraps below zero when
+                // the list starts past the query's base, and only the 32-bit
+                // sum abase + row wraps back to the index
+                const uint32_t arr = abase[j] + row;
+                if (valid)
+                    a.keys_out[(size_t)(pid[j] / a.nprobe) * a.keys_ld + arr] = make_key(sc, arr);
+            } else {
+                tk[j].offer(valid ? make_key(sc, abase[j] + row) : 0, valid);
+            }
         }
     }
+    if constexpr (KEYS) return;
 #pragma unroll
     for (int j = 0; j < QB; ++j) {
         if ((uint32_t)j >= pn) break;
@@ -934,6 +951,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         throw;
     }
     free_all();
+    ivf.read_sizes(s);
     ivf.build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -969,6 +987,7 @@ void ivf_load(IvfIndex &ivf, const float4 *x, uint32_t rows, uint32_t pdims, uin
                        ivf.xr);
     NNDEX_HIP_CHECK(hipGetLastError());
     NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+    ivf.read_sizes(s);
     ivf.build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1040,7 +1059,7 @@ void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(pair_prep_kernel, dim3(grid), dim3(256), 0, s, a.probe, pairs, a.pair_key,
                        a.pair_val);
     hipLaunchKernelGGL(pair_base_kernel, dim3((a.nq + 255) / 256), dim3(256), 0, s, a.probe, a.nq,
-                       a.nprobe, a.offsets, a.pair_base);
+                       a.nprobe, a.offsets, a.pair_base, a.totals);
     size_t bytes = a.sort_bytes;
     NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(a.sort_tmp, bytes, a.pair_key, a.sorted_key,
                                               a.pair_val, a.sorted_pair, pairs, 0,
@@ -1059,6 +1078,16 @@ void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
     const size_t gmax = (pairs + a.qb - 1) / a.qb + (a.nlist < pairs ? a.nlist : pairs);
     const size_t waves = gmax * a.nchunk;
     const uint32_t sgrid = static_cast<uint32_t>((waves + 3) / 4);
+    if (a.keys_out) {
+        if (a.qb == 8)
+            hipLaunchKernelGGL((ivf_list_scan_kernel<8, 128, true>), dim3(sgrid), dim3(256), 0, s, a);
+        else if (a.qb == 4)
+            hipLaunchKernelGGL((ivf_list_scan_kernel<4, 128, true>), dim3(sgrid), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((ivf_list_scan_kernel<1, 128, true>), dim3(sgrid), dim3(256), 0, s, a);
+        NNDEX_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const uint32_t cw = ivf_cw(a.kk);
 #define NNDEX_LIST_CASE(QBV, C)                                                          \
     if (a.qb == QBV && cw == C) {                                                        \
@@ -1092,6 +1121,131 @@ void ivf_scan(const IvfScanArgs &a, hipStream_t s) {
         break;
     }
     NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+// ---- large k ----------------------------------------------------------------
+
+// Original row of arrival index `arr` of query q: the probed list holding it
+// (the last probe whose base is <= arr), then the reordered -> original map.
+__device__ inline uint32_t ivf_arrival_orig(const IvfScanArgs &a, uint32_t q, uint32_t arr) {
+    const uint32_t *pb = a.pair_base + (size_t)q * a.nprobe;
+    uint32_t lo = 0, hi = a.nprobe;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pb[mid] <= arr)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t c = (uint32_t)a.probe[(size_t)q * a.nprobe + lo];
+    return a.orig[a.offsets[c] + (arr - pb[lo])];
+}
+
+// One block per query, over its sorted top min(2k - 1, n) keys (score,
+// arrival): the rows the reference heap keeps (as ivf_merge_kernel: G, then
+// the tied rows up to t*, the e lowest original rows of the window evicted),
+// re-keyed by original row into cand[q], their count into cnt[q].
+__global__ __launch_bounds__(1024) void ivf_finish_a_kernel(IvfScanArgs a, SelectBuffers b,
+                                                            uint32_t k) {
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t so[2];
+    __shared__ uint32_t g_s, u_s, e_s, tie_s, pos_s;
+    const uint32_t q = blockIdx.x;
+    const uint64_t *L = b.sorted + (size_t)q * b.ksel;
+    const uint32_t nsel = b.cnt[q] < b.ksel ? b.cnt[q] : b.ksel;
+    const uint32_t m = k < nsel ? k : nsel;
+    if (threadIdx.x == 0) {
+        g_s = 0;
+        u_s = 0;
+        e_s = 0;
+        tie_s = 0;
+        pos_s = 0;
+        if (k >= 2 && k < nsel && (uint32_t)(L[k] >> 32) == (uint32_t)(L[k - 1] >> 32)) {
+            const uint32_t sh = (uint32_t)(L[k - 1] >> 32);
+            uint32_t lo = 0, hi = nsel;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                if ((uint32_t)(L[mid] >> 32) > sh) lo = mid + 1; else hi = mid;
+            }
+            g_s = lo;
+            lo = 0;
+            hi = nsel;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                if ((uint32_t)(L[mid] >> 32) >= sh) lo = mid + 1; else hi = mid;
+            }
+            u_s = lo;
+            tie_s = 1;
+        }
+    }
+    __syncthreads();
+    const uint32_t g = g_s;
+    if (tie_s) {
+        // t*: the k-th smallest arrival among the rows >= s
+        const uint32_t tstar =
+            block_select_small(u_s, k, [&](uint32_t i) { return key_row(L[i]); }, h, so);
+        uint32_t local = 0;
+        for (uint32_t i = threadIdx.x; i < g; i += blockDim.x) local += key_row(L[i]) > tstar;
+        if (local) atomicAdd(&e_s, local);
+        __syncthreads();
+    }
+    const uint32_t e = e_s, w = m + e;
+    // the e lowest original rows of the tied window L[g, w) are evicted
+    uint32_t drop_max = 0;
+    if (e) {
+        drop_max = block_select_small(
+            w - g, e, [&](uint32_t i) { return ivf_arrival_orig(a, q, key_row(L[g + i])); }, h,
+            so);
+    }
+    uint64_t *out = b.cand + (size_t)q * b.ksel;
+    for (uint32_t i = threadIdx.x; i < w; i += blockDim.x) {
+        const uint32_t id = ivf_arrival_orig(a, q, key_row(L[i]));
+        if (i < g || e == 0 || id > drop_max) {
+            const uint32_t p = atomicAdd(&pos_s, 1u);
+            if (p < b.ksel) out[p] = (L[i] & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - id);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) b.cnt[q] = pos_s < b.ksel ? pos_s : b.ksel;  // = m
+}
+
+__global__ __launch_bounds__(256) void ivf_finish_b_kernel(SelectBuffers b, uint32_t k,
+                                                           uint64_t *__restrict__ out_idx,
+                                                           float *__restrict__ out_sim,
+                                                           uint64_t *__restrict__ out_len) {
+    const uint32_t q = blockIdx.x;
+    const uint32_t m = b.cnt[q] < k ? b.cnt[q] : k;
+    const uint64_t *L = b.sorted + (size_t)q * b.ksel;
+    for (uint32_t i = threadIdx.x; i < k; i += 256) {
+        const uint64_t key = i < m ? L[i] : 0;
+        out_idx[(size_t)q * k + i] = key ? key_row(key) : 0;
+        out_sim[(size_t)q * k + i] = key ? key_sim(key) : 0.f;
+    }
+    if (threadIdx.x == 0) out_len[q] = m;
+}
+
+void ivf_finish_large_k(const IvfScanArgs &a, const SelectBuffers &b, uint32_t k,
+                        hipStream_t s) {
+    if (b.nq == 0) return;
+    if (!b.n) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the IVF selection needs per-query counts");
+    hipLaunchKernelGGL(ivf_finish_a_kernel, dim3(b.nq), dim3(1024), 0, s, a, b, k);
+    NNDEX_HIP_CHECK(hipGetLastError());
+    select_sort(b, s);
+    hipLaunchKernelGGL(ivf_finish_b_kernel, dim3(b.nq), dim3(256), 0, s, b, k, a.out_idx,
+                       a.out_sim, a.out_len);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+void IvfIndex::read_sizes(hipStream_t s) {
+    std::vector<uint32_t> off(nlist + 1);
+    NNDEX_HIP_CHECK(hipMemcpyAsync(off.data(), offsets, (nlist + 1) * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s));
+    NNDEX_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint64_t> sz(nlist);
+    for (uint32_t c = 0; c < nlist; ++c) sz[c] = off[c + 1] - off[c];
+    std::sort(sz.begin(), sz.end(), [](uint64_t x, uint64_t y) { return x > y; });
+    size_prefix.assign(nlist + 1, 0);
+    for (uint32_t c = 0; c < nlist; ++c) size_prefix[c + 1] = size_prefix[c] + sz[c];
 }
 
 }  // namespace nndex

@@ -7,6 +7,9 @@
 
 #include <cstdint>
 #include <memory>
+#include <vector>
+
+#include "select.hpp"
 
 namespace nndex {
 
@@ -30,6 +33,15 @@ struct IvfIndex {
     uint32_t rechecked = 0;       // rows re-evaluated exactly, summed over iterations
     double build_ms = 0.0;
     std::unique_ptr<GpuIndex> cindex;  // centroid index used for probing
+    // host: prefix sums of the list sizes sorted descending, so
+    // max_probe_rows(n) bounds the rows of any n probed lists
+    std::vector<uint64_t> size_prefix;
+    size_t max_probe_rows(size_t nprobe) const {
+        if (size_prefix.empty()) return rows;
+        return size_prefix[nprobe < size_prefix.size() ? nprobe : size_prefix.size() - 1];
+    }
+    // fills size_prefix from the device offsets (after a build or load)
+    void read_sizes(hipStream_t s);
     // ApproxIndex::provides_speedup (approx.rs:34-36)
     bool provides_speedup() const { return rows >= kMinRowsForIvf && nlist >= 4; }
     ~IvfIndex();
@@ -61,11 +73,29 @@ struct IvfScanArgs {
     uint64_t *out_idx;          // [nq][k]
     float *out_sim;             // [nq][k]
     uint64_t *out_len;          // [nq]
+    // large-k form: every probed row's key (score, arrival index) is written
+    // to keys_out[q][arrival] (no per-list top-k, no merge) and each query's
+    // probed row count to totals[q]
+    uint64_t *keys_out;
+    size_t keys_ld;
+    uint32_t *totals;           // [nq] (may be null outside the large-k form)
 };
 
+// Whether k results need the large-k form: the list scan's LDS lists keep
+// 2k - 1 keys (the reference heap's tie rule) only up to 960.
+inline bool ivf_large_k(size_t k) { return 2 * k - 1 > 960; }
+
+// Large-k results from the selection over keys_out (b.sorted holds each
+// query's top min(2k - 1, totals[q]) keys): the reference heap's choice among
+// rows tied at the k-th score over arrival order, the lowest original rows
+// evicted first, results in (score desc, original row asc) order, and
+// out_len[q] = min(k, totals[q]).  Uses b.cand and b.cnt as scratch.
+void ivf_finish_large_k(const IvfScanArgs &a, const SelectBuffers &b, uint32_t k,
+                        hipStream_t s);
+
 // Keys the list scan keeps per (query, list, chunk) for k results: 2k - 1, so
-// the merge can apply the reference heap's choice among tied rows (k <= 480),
-// else k.
+// the merge can apply the reference heap's choice among tied rows (k <= 480;
+// larger k take the large-k form).
 uint32_t ivf_select_k(size_t k);
 // Launch shape of the list scan for a batch: pairs per wavefront and row
 // chunks per list (enough wavefronts to fill the device, a bounded merge).

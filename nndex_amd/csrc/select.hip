@@ -85,8 +85,10 @@ void launch_score_keys(const float4 *x, uint32_t rows, uint32_t d4, const float 
 
 // ---- radix select -------------------------------------------------------------
 
+// Candidates of query q (never past its key array).
 __device__ inline uint32_t sel_count(const SelectBuffers &b, uint32_t q) {
-    return b.n ? b.n[q] : b.n_all;
+    const uint32_t n = b.n ? b.n[q] : b.n_all;
+    return n < b.ld ? n : static_cast<uint32_t>(b.ld);
 }
 
 // Keys of the pass: those whose bits above `shift` equal the prefix's.
@@ -153,6 +155,10 @@ __global__ __launch_bounds__(256) void select_pick_kernel(SelectBuffers b) {
         sum += c[j];
     }
     part[t] = sum;
+    if (t == 0) {
+        sel_bin = 0;
+        sel_above = 0;
+    }
     __syncthreads();
     for (uint32_t off = 1; off < 256; off <<= 1) {
         const uint32_t v = t >= off ? part[t - off] : 0u;
@@ -246,6 +252,11 @@ void select_top(const SelectBuffers &b, hipStream_t s) {
     }
     hipLaunchKernelGGL(select_compact_kernel, dim3(bx, b.nq), dim3(256), 0, s, b);
     NNDEX_HIP_CHECK(hipGetLastError());
+    select_sort(b, s);
+}
+
+void select_sort(const SelectBuffers &b, hipStream_t s) {
+    if (b.nq == 0) return;
     size_t bytes = b.sort_bytes;
     if (!b.n) {
         // every query selects min(ksel, n_all) keys: one device-wide sort each
@@ -264,42 +275,6 @@ void select_top(const SelectBuffers &b, hipStream_t s) {
 }
 
 // ---- the reference heap's tie rule over a sorted canonical list -------------
-
-// Rank-th smallest (1-based) of the 32-bit values val(i), i < n, over the
-// block: four passes of 8 bits with an LDS histogram.  Every thread gets it.
-template <typename Val>
-__device__ uint32_t block_select_small(uint32_t n, uint32_t rank, Val val, uint32_t *h,
-                                       uint32_t *shared_out) {
-    uint32_t prefix = 0, remaining = rank;
-    for (int pass = 0; pass < 4; ++pass) {
-        const uint32_t shift = 24 - 8 * pass;
-        for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) h[i] = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint32_t v = val(i);
-            if (pass == 0 || (v >> (shift + 8)) == (prefix >> (shift + 8)))
-                atomicAdd(&h[(v >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t cum = 0, bin = 255;
-            for (uint32_t j = 0; j < 256; ++j) {
-                if (cum + h[j] >= remaining) {
-                    bin = j;
-                    break;
-                }
-                cum += h[j];
-            }
-            shared_out[0] = prefix | (bin << shift);
-            shared_out[1] = remaining - cum;
-        }
-        __syncthreads();
-        prefix = shared_out[0];
-        remaining = shared_out[1];
-        __syncthreads();
-    }
-    return prefix;
-}
 
 // One block per query.  With s the k-th score: if rows tie at s across the
 // k-th place, the reference's heap (rows pushed in row order, the lowest row

@@ -43,6 +43,42 @@ struct SelectBuffers {
 // Histogram bins of a radix pass.
 constexpr uint32_t kSelectBins = 2048;
 
+// Rank-th smallest (1-based) of the 32-bit values val(i), i < n, over the
+// block: four passes of 8 bits with an LDS histogram.  Every thread gets it.
+template <typename Val>
+__device__ uint32_t block_select_small(uint32_t n, uint32_t rank, Val val, uint32_t *h,
+                                       uint32_t *shared_out) {
+    uint32_t prefix = 0, remaining = rank;
+    for (int pass = 0; pass < 4; ++pass) {
+        const uint32_t shift = 24 - 8 * pass;
+        for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint32_t v = val(i);
+            if (pass == 0 || (v >> (shift + 8)) == (prefix >> (shift + 8)))
+                atomicAdd(&h[(v >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t cum = 0, bin = 255;
+            for (uint32_t j = 0; j < 256; ++j) {
+                if (cum + h[j] >= remaining) {
+                    bin = j;
+                    break;
+                }
+                cum += h[j];
+            }
+            shared_out[0] = prefix | (bin << shift);
+            shared_out[1] = remaining - cum;
+        }
+        __syncthreads();
+        prefix = shared_out[0];
+        remaining = shared_out[1];
+        __syncthreads();
+    }
+    return prefix;
+}
+
 // rocPRIM scratch for sorting nq segments of up to ksel keys.
 size_t select_sort_bytes(size_t nq, size_t ksel);
 
@@ -55,6 +91,9 @@ void launch_score_keys(const float4 *x, uint32_t rows, uint32_t d4, const float 
 // Select the top min(ksel, n_q) keys of every query and sort them
 // descending into b.sorted (b.cnt[q] of them).
 void select_top(const SelectBuffers &b, hipStream_t s);
+
+// The sort stage alone: cand[q][0, min(cnt[q], ksel)) -> sorted, descending.
+void select_sort(const SelectBuffers &b, hipStream_t s);
 
 // Exact-search results from sorted canonical lists: k results per query;
 // with `heap_ties` (each list then holds the top min(2k - 1, rows) keys) the

@@ -280,3 +280,60 @@ def test_ivf_save_and_load_skip_the_build(orc, tmp_path):
     ridx, rsim = ref.search_gpu(orc.normalize_vector(qs[0]), 7, 6)
     np.testing.assert_array_equal(b_one[0], ridx)
     assert np.array_equal(b_one[1].view(np.uint32), rsim.view(np.uint32))
+
+
+@pytest.mark.parametrize("k,nprobe", [(481, 20), (1000, 20), (3000, 24), (1000, 2)])
+def test_ivf_large_k_equals_oracle(orc, k, nprobe):
+    """k beyond the list scan's LDS lists (2k - 1 > 960): every probed row's
+    key, a radix select of the top 2k - 1 per query and the heap's tie rule
+    over arrival order.  nprobe 2 probes fewer rows than k (results = the
+    probed rows).  Single and batch searches equal the oracle bit for bit,
+    padding past the probed rows included."""
+    rows, dims = 30_000, 32
+    _case("ivf_large_k", rows=rows, dims=dims, k=k, nprobe=nprobe)
+    m, rng = _storage(orc, rows, dims, 5 + k, mixture=True)
+    qs = orc.normalize_rows(rng.normal(size=(7, dims)).astype(np.float32), 7, dims)
+    b = BackendIndex(m, approx=True)
+    b.set_ivf(nprobe=nprobe)
+    ref = orc.Ivf(m)
+    idx, sim = b.search(qs[0], k, approx=True)
+    ridx, rsim = ref.search_gpu(qs[0], k, nprobe)
+    np.testing.assert_array_equal(idx, ridx)
+    assert np.array_equal(sim.view(np.uint32), rsim.view(np.uint32))
+    bidx, bsim, lens = b.search_batch(qs, k, approx=True)
+    for i, q in enumerate(qs):
+        ridx, rsim = ref.search_gpu(q, k, nprobe)
+        assert int(lens[i]) == len(ridx)
+        np.testing.assert_array_equal(bidx[i, : lens[i]], ridx)
+        assert np.array_equal(bsim[i, : lens[i]].view(np.uint32), rsim.view(np.uint32))
+        assert np.all(bidx[i, lens[i]:] == 0)
+
+
+def test_ivf_large_k_ties_follow_the_reference_heap(orc):
+    """Duplicated rows spread over several lists: rows tie bit for bit across
+    the k-th place (k = 700, beyond the list scan), where the reference heap
+    keeps rows by probe arrival and evicts the lowest original rows."""
+    _case("ivf_large_k_ties")
+    rng = np.random.default_rng(8)
+    pool = rng.normal(size=(150, 24)).astype(np.float32)
+    m = orc.normalize_rows(pool[rng.integers(0, 150, size=20_000)], 20_000, 24)
+    qs = orc.normalize_rows(pool[rng.integers(0, 150, size=6)] +
+                            0.5 * rng.normal(size=(6, 24)).astype(np.float32), 6, 24)
+    b = BackendIndex(m, approx=True)
+    ref = orc.Ivf(m)
+    shifted = 0
+    for k, nprobe in ((700, 6), (1500, 10)):
+        b.set_ivf(nprobe=nprobe)
+        bidx, bsim, lens = b.search_batch(qs, k, approx=True)
+        for i, q in enumerate(qs):
+            ridx, rsim = ref.search_gpu(q, k, nprobe)
+            assert int(lens[i]) == len(ridx)
+            np.testing.assert_array_equal(bidx[i, : lens[i]], ridx)
+            assert np.array_equal(bsim[i, : lens[i]].view(np.uint32), rsim.view(np.uint32))
+            lists = ref.probe(q, nprobe)
+            off, orig = ref.offsets(), ref.original_indices()
+            rws = [int(orig[r]) for c in lists for r in range(int(off[c]), int(off[c + 1]))]
+            sc = {r: orc.dot_product(m[r], q) for r in rws}
+            canon = sorted(rws, key=lambda r: (-sc[r], r))[:k]
+            shifted += canon != ridx.tolist()
+    assert shifted > 0
