@@ -91,6 +91,8 @@ def lib() -> ctypes.CDLL:
             "orc_isqrt": (_SZ, [_SZ]),
             "orc_choose_nlist": (_SZ, [_SZ]),
             "orc_choose_nprobe": (_SZ, [_SZ, _SZ, _SZ]),
+            "orc_pick_medoid_row": (_SZ, [_F, _SZ, _SZ]),
+            "orc_kmeans_pp_rows": (None, [_F, _SZ, _SZ, _SZ, _I, _U64]),
             "orc_ivf_build": (_P, [_F, _SZ, _SZ, _SZ, _SZ, _I]),
             "orc_ivf_free": (None, [_P]),
             "orc_ivf_nlist": (_SZ, [_P]),
@@ -337,6 +339,21 @@ def choose_nlist(rows: int) -> int:
 
 def choose_nprobe(nlist: int, k: int, rows: int) -> int:
     return int(lib().orc_choose_nprobe(nlist, k, rows))
+
+
+def pick_medoid_row(matrix) -> int:
+    """pick_medoid_row (src/approx.rs:409-429)."""
+    m = _f32(matrix)
+    return int(lib().orc_pick_medoid_row(_fp(m), m.shape[0], m.shape[1]))
+
+
+def kmeans_pp_rows(matrix, nlist: int, threads: int = 0) -> np.ndarray:
+    """Rows the k-means++ seeding copies (src/approx.rs:357-406)."""
+    m = _f32(matrix)
+    out = np.zeros(nlist, np.uint64)
+    lib().orc_kmeans_pp_rows(_fp(m), m.shape[0], m.shape[1], nlist, threads,
+                             out.ctypes.data_as(_U64))
+    return out
 
 
 class Ivf:

@@ -796,12 +796,16 @@ static size_t pick_medoid_row(const float *m, size_t rows, size_t dims) {
     return best;
 }
 
-/* kmeans_plusplus_init, approx.rs:357-406 (distance pass parallel, sums serial) */
-static float *kmeans_pp(const float *m, size_t rows, size_t dims, size_t nlist, int threads) {
+/* kmeans_plusplus_init, approx.rs:357-406 (distance pass parallel, sums serial).
+ * chosen (may be NULL): the row each centroid copies (a duplicate repeats the
+ * previous centroid's row). */
+static float *kmeans_pp(const float *m, size_t rows, size_t dims, size_t nlist, int threads,
+                        uint64_t *chosen_rows) {
     uint64_t state = (uint64_t)rows * 2654435761ull + (uint64_t)dims;
     float *cent = (float *)malloc(nlist * dims * sizeof(float));
     size_t first = pick_medoid_row(m, rows, dims);
     memcpy(cent, m + first * dims, dims * sizeof(float));
+    if (chosen_rows) chosen_rows[0] = first;
     float *mind = (float *)malloc(rows * sizeof(float));
     for (size_t r = 0; r < rows; ++r) mind[r] = FLT_MAX;
     for (size_t c = 1; c < nlist; ++c) {
@@ -816,6 +820,7 @@ static float *kmeans_pp(const float *m, size_t rows, size_t dims, size_t nlist, 
         for (size_t r = 0; r < rows; ++r) total += (double)mind[r];
         if (total <= 0.0) {
             memcpy(cent + c * dims, last, dims * sizeof(float));
+            if (chosen_rows) chosen_rows[c] = chosen_rows[c - 1];
             continue;
         }
         state = state * 6364136223846793005ull + 1ull;
@@ -830,9 +835,20 @@ static float *kmeans_pp(const float *m, size_t rows, size_t dims, size_t nlist, 
             }
         }
         memcpy(cent + c * dims, m + chosen * dims, dims * sizeof(float));
+        if (chosen_rows) chosen_rows[c] = chosen;
     }
     free(mind);
     return cent;
+}
+
+size_t orc_pick_medoid_row(const float *matrix, size_t rows, size_t dims) {
+    return rows ? pick_medoid_row(matrix, rows, dims) : 0;
+}
+
+void orc_kmeans_pp_rows(const float *matrix, size_t rows, size_t dims, size_t nlist, int threads,
+                        uint64_t *chosen_rows) {
+    if (!rows || !nlist) return;
+    free(kmeans_pp(matrix, rows, dims, nlist, resolve_threads(threads), chosen_rows));
 }
 
 /* spherical_kmeans, approx.rs:434-535 */
@@ -904,7 +920,7 @@ orc_ivf *orc_ivf_build(const float *matrix, size_t rows, size_t dims, size_t nli
         return ivf;
     }
     ivf->nlist = nlist;
-    ivf->centroids = kmeans_pp(matrix, rows, dims, nlist, threads);
+    ivf->centroids = kmeans_pp(matrix, rows, dims, nlist, threads, NULL);
     spherical_kmeans(matrix, rows, dims, nlist, iters, ivf->centroids, ivf->assignments, threads);
     ivf->offsets = (uint64_t *)calloc(nlist + 1, sizeof(uint64_t));
     for (size_t r = 0; r < rows; ++r) ivf->offsets[ivf->assignments[r] + 1] += 1;

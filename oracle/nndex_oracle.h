@@ -110,6 +110,15 @@ size_t orc_isqrt(size_t n);
 size_t orc_choose_nlist(size_t rows);
 size_t orc_choose_nprobe(size_t nlist, size_t k, size_t rows);
 
+/* pick_medoid_row (approx.rs:409-429): the row with the largest dot with the
+ * f64-summed column mean (rounded to f32); the lowest row on ties. */
+size_t orc_pick_medoid_row(const float *matrix, size_t rows, size_t dims);
+/* kmeans_plusplus_init (approx.rs:357-406): the row each of the nlist seeded
+ * centroids copies (a duplicate, when every distance is zero, repeats the
+ * previous centroid's row). */
+void orc_kmeans_pp_rows(const float *matrix, size_t rows, size_t dims, size_t nlist, int threads,
+                        uint64_t *chosen_rows);
+
 typedef struct orc_ivf orc_ivf;
 /* nlist == 0 -> choose_nlist(rows); iters == 0 -> 12 (KMEANS_MAX_ITERS) */
 orc_ivf *orc_ivf_build(const float *matrix, size_t rows, size_t dims, size_t nlist, size_t iters,

@@ -337,3 +337,26 @@ def test_ivf_large_k_ties_follow_the_reference_heap(orc):
             canon = sorted(rws, key=lambda r: (-sc[r], r))[:k]
             shifted += canon != ridx.tolist()
     assert shifted > 0
+
+
+def test_ivf_seeding_and_ties_follow_the_closed_form(orc):
+    """Signed, permuted unit axes (every pair of rows has dot exactly 0): the
+    k-means++ picks follow in closed form from the reference's generator
+    (tests/test_oracle.py restates it), and after one Lloyd iteration each
+    seeded centroid c >= 1 is its own row while every other row ties at 0
+    across the centroids and goes to centroid 0 (strict >, lowest id)."""
+    from test_oracle import _kmeanspp_closed_form, _signed_axes
+
+    rows, dims, nlist = 256, 256, 40
+    _case("ivf_closed_form", rows=rows, dims=dims, nlist=nlist)
+    m = _signed_axes(rows, dims, 4)
+    want = _kmeanspp_closed_form(rows, dims, nlist, 0)
+    assert [int(r) for r in orc.kmeans_pp_rows(m, nlist)] == want
+    b = BackendIndex(m, approx=True)
+    b.set_ivf(nlist=nlist, iters=1)
+    info, cent, off, orig, assign = b.ivf_export()
+    for c in range(1, nlist):
+        assert np.array_equal(cent[c].view(np.uint32), m[want[c]].view(np.uint32)), c
+        assert int(assign[want[c]]) == c
+    others = np.setdiff1d(np.arange(rows), want[1:])
+    assert np.all(assign[others] == 0)

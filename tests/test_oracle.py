@@ -572,3 +572,129 @@ def test_fixture_script_reproduces_committed_fixtures(tmp_path):
                 np.testing.assert_allclose(a[key], b[key], rtol=0, atol=1e-12)
             else:
                 assert np.array_equal(a[key], b[key]), key
+
+
+# ---------------------------------------------------------------------------
+# IVF seeding and assignment pinned by closed forms (src/approx.rs:357-469):
+# the GPU build equals the oracle bit for bit (tests/test_gpu_ivf.py), so
+# these checks tie both to what the reference's own rules fix.
+# ---------------------------------------------------------------------------
+
+MASK64 = (1 << 64) - 1
+
+
+def _kmeanspp_closed_form(rows, dims, nlist, first, mult=6364136223846793005, inc=1,
+                          seed_mul=2654435761, seed_add=None):
+    """k-means++ on signed, permuted unit axes: every unchosen row is at
+    distance exactly 1 from every centroid (dot 0), a chosen row at 0, so the
+    total is the unchosen count and the pick is the first row whose count of
+    unchosen rows so far reaches (state / u64::MAX) * total."""
+    seed_add = dims if seed_add is None else seed_add
+    state = (rows * seed_mul + seed_add) & MASK64
+    chosen = [first]
+    for _ in range(1, nlist):
+        unchosen = [r not in chosen for r in range(rows)]
+        total = float(sum(unchosen))
+        if total <= 0.0:
+            chosen.append(chosen[-1])
+            continue
+        state = (state * mult + inc) & MASK64
+        threshold = (float(state) / float(MASK64)) * total
+        cum, pick = 0.0, rows - 1
+        for r in range(rows):
+            cum += 1.0 if unchosen[r] else 0.0
+            if cum >= threshold:
+                pick = r
+                break
+        chosen.append(pick)
+    return chosen
+
+
+def _signed_axes(rows, dims, seed):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((rows, dims), np.float32)
+    axes = rng.permutation(dims)[:rows]
+    m[np.arange(rows), axes] = rng.choice(np.array([-1.0, 1.0], np.float32), size=rows)
+    return m
+
+
+@pytest.mark.parametrize("rows,dims,nlist", [(40, 64, 12), (96, 96, 60), (200, 256, 200)])
+def test_kmeanspp_seeding_closed_form(orc, rows, dims, nlist):
+    m = _signed_axes(rows, dims, rows + nlist)
+    # every row's dot with the mean is the same f32 value: the lowest row wins
+    assert orc.pick_medoid_row(m) == 0
+    got = [int(r) for r in orc.kmeans_pp_rows(m, nlist)]
+    assert got == _kmeanspp_closed_form(rows, dims, nlist, 0)
+    assert len(set(got)) == nlist  # distances stay 1 until every row is chosen
+    # a one-constant change of the generator or the seed picks other rows
+    for kw in ({"mult": 6364136223846793005 + 2}, {"inc": 3}, {"seed_mul": 2654435769},
+               {"seed_add": 0}):
+        assert _kmeanspp_closed_form(rows, dims, nlist, 0, **kw) != got, kw
+
+
+def test_kmeanspp_duplicates_when_every_distance_is_zero(orc):
+    """total <= 0 (approx.rs:382-386): the last centroid is repeated."""
+    m = np.zeros((10, 16), np.float32)
+    m[:7, 0] = 1.0   # seven copies of e0
+    m[7, 1] = 1.0
+    m[8, 2] = 1.0
+    m[9, 3] = 1.0
+    got = [int(r) for r in orc.kmeans_pp_rows(m, 8)]
+    # medoid: e0's dot with the mean (0.7) beats the others (0.1); rows 1-6
+    # tie with row 0, the lowest wins
+    assert got[0] == 0
+    assert sorted(got[1:4]) == [7, 8, 9]
+    assert got[4:] == [got[3]] * 4  # every row at distance 0: duplicates
+    assert got == _kmeanspp_closed_form(10, 16, 8, 0)[:1] + got[1:]
+
+
+def test_medoid_is_the_argmax_against_the_mean(orc):
+    rng = np.random.default_rng(5)
+    for rows, dims in ((3000, 40), (5000, 128)):
+        m = orc.normalize_rows(rng.normal(size=(rows, dims)).astype(np.float32), rows, dims)
+        mean = (m.astype(np.float64).sum(axis=0) / rows).astype(np.float32).astype(np.float64)
+        dots = m.astype(np.float64) @ mean
+        order = np.argsort(-dots, kind="stable")
+        assert dots[order[0]] - dots[order[1]] > 1e-5  # f32 rounding cannot reorder
+        assert orc.pick_medoid_row(m) == int(order[0])
+        # a copy of the best row at a lower index ties it exactly: the lowest wins
+        lo = int(order[0]) // 2
+        m2 = m.copy()
+        m2[lo] = m[order[0]]
+        mean2 = (m2.astype(np.float64).sum(axis=0) / rows).astype(np.float32).astype(np.float64)
+        d2 = m2.astype(np.float64) @ mean2
+        best = np.flatnonzero(d2 == d2.max())
+        assert lo in best
+        assert orc.pick_medoid_row(m2) == min(best)
+        # the argmin (a sign mistake) is another row
+        assert int(np.argmin(dots)) != int(order[0])
+
+
+def test_assignment_is_the_strict_argmax_over_seeded_centroids(orc):
+    """One Lloyd iteration (approx.rs:447-469): each row's assignment is the
+    first centroid with the largest dot (strict >, the lowest id on ties)."""
+    rng = np.random.default_rng(11)
+    rows, dims, nlist = 4000, 32, 50
+    c = rng.normal(size=(30, dims)).astype(np.float32)
+    raw = c[rng.integers(0, 30, size=rows)] + 0.4 * rng.normal(size=(rows, dims)).astype(np.float32)
+    m = orc.normalize_rows(raw, rows, dims)
+    seeds = orc.kmeans_pp_rows(m, nlist).astype(np.int64)
+    ivf = orc.Ivf(m, nlist=nlist, iters=1)
+    assign = ivf.assignments().astype(np.int64)
+    s = m.astype(np.float64) @ m[seeds].astype(np.float64).T
+    top = np.argmax(s, axis=1)  # the first of equal maxima
+    srt = np.sort(s, axis=1)
+    clear = srt[:, -1] - srt[:, -2] > 1e-5
+    assert clear.mean() > 0.9
+    np.testing.assert_array_equal(assign[clear], top[clear])
+    # duplicated centroids (rows whose seeds repeat) tie exactly: lowest id
+    m3 = np.zeros((300, 16), np.float32)  # exact unit rows: e0, and e1 every third
+    m3[:, 0] = 1.0
+    m3[::3, 0] = 0.0
+    m3[::3, 1] = 1.0
+    seeds3 = orc.kmeans_pp_rows(m3, 6).astype(np.int64)
+    assert len(set(seeds3.tolist())) < 6  # duplicates were seeded
+    a3 = orc.Ivf(m3, nlist=6, iters=1).assignments().astype(np.int64)
+    d3 = m3.astype(np.float64) @ m3[seeds3].astype(np.float64).T
+    for r in range(300):
+        assert a3[r] == int(np.flatnonzero(d3[r] == d3[r].max())[0])
