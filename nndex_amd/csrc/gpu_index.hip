@@ -1178,8 +1178,8 @@ const IvfIndex &GpuIndex::load_ivf(size_t nlist, const float *centroids, const u
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
     if (nlist < 2 || nlist > rows_ || nlist > 16384)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "nlist must be in [2, min(rows, 16384)]");
-    if (pdims_ > 1024)
-        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF supports padded dims <= 1024");
+    if (pdims_ > 16384)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF supports padded dims <= 16384");
     for (size_t i = 0; i < nlist * pdims_; ++i)
         if (!std::isfinite(centroids[i]))
             throw Error(NNDEX_ERR_INVALID_ARGUMENT, "IVF centroids must be finite");

@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <vector>
 
@@ -86,24 +87,24 @@ __device__ inline uint64_t block_max_u64(uint64_t v, uint64_t *red) {
 __global__ __launch_bounds__(256) void colsum_kernel(const float4 *__restrict__ x, uint32_t rows,
                                                      uint32_t d4, uint32_t rows_per_block,
                                                      double *__restrict__ partial) {
-    const uint32_t c = threadIdx.x;  // float4 chunk
-    if (c >= d4) return;
     const uint32_t r0 = blockIdx.x * rows_per_block;
     const uint32_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (uint32_t c = threadIdx.x; c < d4; c += blockDim.x) {  // float4 chunk
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll 4
-    for (uint32_t r = r0; r < r1; ++r) {
-        const float4 v = x[((size_t)(r / 64) * d4 + c) * 64 + r % 64];
-        s0 += v.x;
-        s1 += v.y;
-        s2 += v.z;
-        s3 += v.w;
+        for (uint32_t r = r0; r < r1; ++r) {
+            const float4 v = x[((size_t)(r / 64) * d4 + c) * 64 + r % 64];
+            s0 += v.x;
+            s1 += v.y;
+            s2 += v.z;
+            s3 += v.w;
+        }
+        double *p = partial + (size_t)blockIdx.x * d4 * 4 + (size_t)c * 4;
+        p[0] = s0;
+        p[1] = s1;
+        p[2] = s2;
+        p[3] = s3;
     }
-    double *p = partial + (size_t)blockIdx.x * d4 * 4 + c * 4;
-    p[0] = s0;
-    p[1] = s1;
-    p[2] = s2;
-    p[3] = s3;
 }
 
 __global__ __launch_bounds__(256) void colmean_kernel(const double *__restrict__ partial,
@@ -394,20 +395,22 @@ __global__ __launch_bounds__(256) void offsets_kernel(const uint32_t *__restrict
 }
 
 // Centroid update (spherical_kmeans, approx.rs:490-531): one block per
-// cluster; thread c owns float4 chunk c and sums the cluster's rows in row
-// order in f64; the squared norm of the mean is summed in dimension order.
-__global__ __launch_bounds__(256) void centroid_update_kernel(const float4 *__restrict__ x,
-                                                              uint32_t d4,
-                                                              const uint32_t *__restrict__ sorted_rows,
-                                                              const uint32_t *__restrict__ offsets,
-                                                              float4 *__restrict__ centroids) {
-    __shared__ double scaled[1024];
+// cluster; each thread owns float4 chunks t, t + blockDim, ... and sums the
+// cluster's rows in row order in f64; the squared norm of the mean is summed
+// in dimension order.  Dynamic LDS: 4 d4 doubles.
+__global__ __launch_bounds__(1024) void centroid_update_kernel(const float4 *__restrict__ x,
+                                                               uint32_t d4,
+                                                               const uint32_t *__restrict__ sorted_rows,
+                                                               const uint32_t *__restrict__ offsets,
+                                                               float4 *__restrict__ centroids) {
+    extern __shared__ double scaled[];
     __shared__ double nsq;
-    const uint32_t cl = blockIdx.x, t = threadIdx.x;
+    const uint32_t cl = blockIdx.x;
     const uint32_t r0 = offsets[cl], r1 = offsets[cl + 1];
     const uint32_t count = r1 - r0;
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    if (t < d4) {
+    const double inv_count = count ? 1.0 / (double)count : 0.0;
+    for (uint32_t t = threadIdx.x; t < d4; t += blockDim.x) {
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll 4
         for (uint32_t i = r0; i < r1; ++i) {
             const uint32_t r = sorted_rows[i];
@@ -417,28 +420,27 @@ __global__ __launch_bounds__(256) void centroid_update_kernel(const float4 *__re
             s2 += v.z;
             s3 += v.w;
         }
-    }
-    const double inv_count = count ? 1.0 / (double)count : 0.0;
-    if (t < d4) {
         scaled[4 * t + 0] = s0 * inv_count;
         scaled[4 * t + 1] = s1 * inv_count;
         scaled[4 * t + 2] = s2 * inv_count;
         scaled[4 * t + 3] = s3 * inv_count;
     }
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
         double acc = 0;
         for (uint32_t i = 0; i < 4 * d4; ++i) acc += scaled[i] * scaled[i];
         nsq = acc;
     }
     __syncthreads();
-    if (t < d4) {
+    const double inv_norm = nsq > 1e-30 ? 1.0 / sqrt(nsq) : 1.0;
+    for (uint32_t t = threadIdx.x; t < d4; t += blockDim.x) {
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         if (count) {
-            const double inv_norm = nsq > 1e-30 ? 1.0 / sqrt(nsq) : 1.0;
-            out = make_float4((float)(s0 * inv_count * inv_norm), (float)(s1 * inv_count * inv_norm),
-                              (float)(s2 * inv_count * inv_norm),
-                              (float)(s3 * inv_count * inv_norm));
+            // (sum / count) / norm: the sums scaled once, as the reference
+            out = make_float4((float)(scaled[4 * t + 0] * inv_norm),
+                              (float)(scaled[4 * t + 1] * inv_norm),
+                              (float)(scaled[4 * t + 2] * inv_norm),
+                              (float)(scaled[4 * t + 3] * inv_norm));
         }
         centroids[(size_t)cl * d4 + t] = out;
     }
@@ -477,6 +479,90 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float4 *__restrict__
         }
         best = block_max_u64(best, red);
         if (threadIdx.x == 0) assign[row] = key_row(best);
+    }
+}
+
+// Exact row argmax for rows too wide for the MFMA assignment (padded dims
+// above 1024): a tile of 64 rows against every centroid, 64 centroids at a
+// time, chunks of 8 float4 of both staged in LDS.  Thread (ty, tx) = (t / 16,
+// t % 16) owns rows ty + 16 i and centroids tx + 16 j (i, j < 4) and keeps,
+// per pair, the reference dot's four FMA chains in registers across chunks
+// (lane j accumulates elements 4c + j in chunk order, then (l0 + l1) +
+// (l2 + l3)), so every score is the reference's f32 value and the argmax
+// needs no re-check: the highest score, the lowest centroid on ties (strict
+// >, approx.rs:447-469).
+constexpr uint32_t kWideChunk = 8;
+__global__ __launch_bounds__(256) void assign_exact_kernel(const float4 *__restrict__ x,
+                                                           uint32_t rows, uint32_t d4,
+                                                           const float4 *__restrict__ centroids,
+                                                           uint32_t nlist,
+                                                           uint32_t *__restrict__ assign) {
+    __shared__ float4 xs[kWideChunk][64];
+    __shared__ float4 cs[kWideChunk][64];
+    const uint32_t t = threadIdx.x, ty = t / 16, tx = t % 16;
+    const uint32_t tile = blockIdx.x;
+    uint64_t best[4] = {0, 0, 0, 0};
+    for (uint32_t c0 = 0; c0 < nlist; c0 += 64) {
+        float acc[4][4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j][0] = acc[i][j][1] = acc[i][j][2] = acc[i][j][3] = 0.f;
+        for (uint32_t k0 = 0; k0 < d4; k0 += kWideChunk) {
+            const uint32_t kn = d4 - k0 < kWideChunk ? d4 - k0 : kWideChunk;
+            __syncthreads();
+            for (uint32_t u = t; u < kWideChunk * 64; u += 256) {
+                const uint32_t kc = u / 64, r = u % 64;
+                float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), cv = xv;
+                if (kc < kn) {
+                    // tiled rows: chunk k0 + kc of the tile's 64 rows (padding
+                    // rows of the last tile are zero)
+                    xv = x[((size_t)tile * d4 + k0 + kc) * 64 + r];
+                    if (c0 + r < nlist) cv = centroids[(size_t)(c0 + r) * d4 + k0 + kc];
+                }
+                xs[kc][r] = xv;
+                cs[kc][r] = cv;
+            }
+            __syncthreads();
+            for (uint32_t kc = 0; kc < kn; ++kc) {
+                float4 xa[4], cb[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) xa[i] = xs[kc][ty + 16 * i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cb[j] = cs[kc][tx + 16 * j];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        acc[i][j][0] = __builtin_fmaf(xa[i].x, cb[j].x, acc[i][j][0]);
+                        acc[i][j][1] = __builtin_fmaf(xa[i].y, cb[j].y, acc[i][j][1]);
+                        acc[i][j][2] = __builtin_fmaf(xa[i].z, cb[j].z, acc[i][j][2]);
+                        acc[i][j][3] = __builtin_fmaf(xa[i].w, cb[j].w, acc[i][j][3]);
+                    }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t cid = c0 + tx + 16 * j;
+                if (cid < nlist) {
+                    const float sc = (acc[i][j][0] + acc[i][j][1]) + (acc[i][j][2] + acc[i][j][3]);
+                    const uint64_t key = argmax_key(sc, cid);
+                    best[i] = key > best[i] ? key : best[i];
+                }
+            }
+    }
+    // the 16 threads of a row group (consecutive lanes) hold its candidates
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint64_t b = best[i];
+        for (int off = 1; off < 16; off <<= 1) {
+            const uint64_t o = __shfl_xor(b, off);
+            b = o > b ? o : b;
+        }
+        const uint32_t row = tile * 64 + ty + 16 * i;
+        if (tx == 0 && row < rows) assign[row] = key_row(b);
     }
 }
 
@@ -780,9 +866,12 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                uint32_t ncu, hipStream_t s) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t d4 = pdims / 4, d8 = pdims / 8;
-    const bool bf16 = pdims > 512;
-    if (pdims > 1024)
-        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF build supports padded dims <= 1024");
+    // assignment: FP32 MFMA up to 512 padded dims, BF16 MFMA up to 1024 (both
+    // with the exact re-check), the exact tiled kernel above
+    const bool wide = pdims > 1024;
+    const bool bf16 = pdims > 512 && !wide;
+    if (pdims > 16384)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "the GPU IVF build supports padded dims <= 16384");
     if (bf16 && !xbf16)
         throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the BF16 copy is required above 512 dims");
     if (nlist < 2 || nlist > rows || nlist > 16384)
@@ -890,6 +979,20 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                                pdims, 0, tol);
         hipLaunchKernelGGL(iota_kernel, dim3(1024), dim3(256), 0, s, rows_iota, rows);
         const size_t hist_lds = (size_t)nlist * sizeof(uint32_t);
+        // centroid update: a thread per float4 chunk (up to 1024), the means
+        // of the chunks in LDS
+        const uint32_t ublock = d4 < 1024 ? (d4 + 63) / 64 * 64 : 1024;
+        const size_t upd_lds = (size_t)4 * d4 * sizeof(double);
+        if (upd_lds > 64 * 1024) {
+            static std::once_flag once;
+            static hipError_t err = hipSuccess;
+            std::call_once(once, [] {
+                err = hipFuncSetAttribute((const void *)centroid_update_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(4 * 4096 * sizeof(double)));
+            });
+            NNDEX_HIP_CHECK(err);
+        }
         std::vector<uint32_t> rechecked(iters, 0);
         for (uint32_t it = 0; it < iters; ++it) {
             launch_zero_u32(recheck_count, 1, s);
@@ -899,28 +1002,34 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                 hipLaunchKernelGGL(assign_tol_kernel, dim3(1), dim3(1), 0, s, cerr, x_err, x_max,
                                    cmax, pdims, 1, tol);
             }
-            MfmaScanArgs m{};
-            m.x = bf16 ? xbf16 : reinterpret_cast<const uint4 *>(x);
-            m.rows = rows;
-            m.rows_padded = static_cast<uint32_t>(prow);
-            m.dch = bf16 ? d8 : d4;
-            m.q = bf16 ? cbf16 : reinterpret_cast<const uint4 *>(ivf.centroids);
-            m.q_stride = m.dch;
-            m.nq = nlist;
-            m.mode = 2;
-            m.assign = ivf.assign;
-            m.tol = tol;
-            m.recheck = recheck;
-            m.recheck_count = recheck_count;
-            const uint32_t br = mfma_block_rows(bf16, m.dch, true);
-            uint32_t grid = (m.rows_padded + br - 1) / br;
-            if (grid > ncu) grid = ncu;
-            launch_mfma_scan(m, bf16, grid, s);
-            hipLaunchKernelGGL(recheck_kernel, dim3(ncu * 4), dim3(256), 0, s, x, d4,
-                               reinterpret_cast<const float4 *>(ivf.centroids), nlist, recheck,
-                               recheck_count, ivf.assign);
-            NNDEX_HIP_CHECK(hipMemcpyAsync(&rechecked[it], recheck_count, sizeof(uint32_t),
-                                           hipMemcpyDeviceToHost, s));
+            if (wide) {
+                hipLaunchKernelGGL(assign_exact_kernel, dim3((rows + 63) / 64), dim3(256), 0, s, x,
+                                   rows, d4, reinterpret_cast<const float4 *>(ivf.centroids),
+                                   nlist, ivf.assign);
+            } else {
+                MfmaScanArgs m{};
+                m.x = bf16 ? xbf16 : reinterpret_cast<const uint4 *>(x);
+                m.rows = rows;
+                m.rows_padded = static_cast<uint32_t>(prow);
+                m.dch = bf16 ? d8 : d4;
+                m.q = bf16 ? cbf16 : reinterpret_cast<const uint4 *>(ivf.centroids);
+                m.q_stride = m.dch;
+                m.nq = nlist;
+                m.mode = 2;
+                m.assign = ivf.assign;
+                m.tol = tol;
+                m.recheck = recheck;
+                m.recheck_count = recheck_count;
+                const uint32_t br = mfma_block_rows(bf16, m.dch, true);
+                uint32_t grid = (m.rows_padded + br - 1) / br;
+                if (grid > ncu) grid = ncu;
+                launch_mfma_scan(m, bf16, grid, s);
+                hipLaunchKernelGGL(recheck_kernel, dim3(ncu * 4), dim3(256), 0, s, x, d4,
+                                   reinterpret_cast<const float4 *>(ivf.centroids), nlist, recheck,
+                                   recheck_count, ivf.assign);
+                NNDEX_HIP_CHECK(hipMemcpyAsync(&rechecked[it], recheck_count, sizeof(uint32_t),
+                                               hipMemcpyDeviceToHost, s));
+            }
             launch_zero_u32(counts, nlist, s);
             hipLaunchKernelGGL(histogram_kernel, dim3(256), dim3(256), hist_lds, s, ivf.assign,
                                rows, nlist, counts);
@@ -929,9 +1038,8 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
             NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ivf.assign,
                                                       sorted_keys, rows_iota, ivf.orig, rows, 0,
                                                       bits, s));
-            const uint32_t ublock = (d4 + 63) / 64 * 64;
-            hipLaunchKernelGGL(centroid_update_kernel, dim3(nlist), dim3(ublock), 0, s, x, d4,
-                               ivf.orig, ivf.offsets, cent);
+            hipLaunchKernelGGL(centroid_update_kernel, dim3(nlist), dim3(ublock), upd_lds, s, x,
+                               d4, ivf.orig, ivf.offsets, cent);
             NNDEX_HIP_CHECK(hipGetLastError());
         }
         // reorder by the last assignment (approx.rs:64-89); orig and offsets

@@ -46,6 +46,8 @@ def _storage(orc, rows, dims, seed, mixture=False):
     (6000, 32, 300, 2),    # many small clusters, some may empty
     (3000, 600, 0, 3),     # above 512 padded dims: BF16 assignment + exact re-check
     (2000, 1000, 40, 2),
+    (3000, 1536, 0, 3),    # above 1024 padded dims: the exact tiled assignment
+    (2500, 3072, 40, 2),
 ])
 def test_ivf_build_equals_oracle(orc, rows, dims, nlist, iters):
     _case("ivf_build", rows=rows, dims=dims, nlist=nlist, iters=iters)
@@ -71,6 +73,8 @@ def test_ivf_build_equals_oracle(orc, rows, dims, nlist, iters):
                "found yet (the same library passed it before), so it stays off the GPU")),
     (4096, 128, 1, 1),
     (6000, 32, 100, 0),
+    (3000, 1536, 10, 0),
+    (2600, 3072, 7, 4),
 ])
 def test_ivf_search_equals_oracle(orc, rows, dims, k, nprobe):
     """Approximate single and batch search through the backend boundary equal
