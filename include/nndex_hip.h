@@ -347,6 +347,18 @@ void nndex_server_free(nndex_server *server);
  * nprobe = 0 picks the reference's choose_nprobe for k (IVF only). */
 nndex_status nndex_server_submit(nndex_server *server, const float *queries, size_t query_rows,
                                  size_t k, int32_t mode, size_t nprobe, uint64_t *ticket);
+/* Queue `query_rows` raw rows of `dims` values each (dims rounded up to a
+ * multiple of 8 must equal the index's padded dims), as the front-end's
+ * search_batch receives them (src/lib.rs:372-402): normalize = 1 normalises
+ * each row on the device as normalize_rows does (src/lib.rs:461-553), 0 only
+ * checks and pads (ensure_finite + pad_rows, for an index of normalized
+ * rows).  The checks run on the device with the batch; a row that fails them
+ * fails its request only, reported by nndex_server_wait as
+ * NNDEX_ERR_NON_FINITE_INPUT or NNDEX_ERR_ZERO_NORM.  Raw requests batch with
+ * raw requests of the same dims and normalize. */
+nndex_status nndex_server_submit_raw(nndex_server *server, const float *queries,
+                                     size_t query_rows, size_t dims, int32_t normalize, size_t k,
+                                     int32_t mode, size_t nprobe, uint64_t *ticket);
 /* Block until the ticket's request is done; copy query_rows x min(k, rows)
  * indices and similarities and query_rows result counts (any may be NULL). */
 nndex_status nndex_server_wait(nndex_server *server, uint64_t ticket, uint64_t *out_indices,

@@ -182,6 +182,7 @@ SIGNATURES = {
     "nndex_server_new": (_ST, [_P, ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
     "nndex_server_free": (None, [_P]),
     "nndex_server_submit": (_ST, [_P, _FP, _SZ, _SZ, _I32, _SZ, _U64P]),
+    "nndex_server_submit_raw": (_ST, [_P, _FP, _SZ, _SZ, _I32, _SZ, _I32, _SZ, _U64P]),
     "nndex_server_wait": (_ST, [_P, ctypes.c_uint64, _U64P, _FP, _U64P,
                                 ctypes.POINTER(ctypes.c_double)]),
     "nndex_server_get_stats": (_ST, [_P, ctypes.POINTER(ServerStats)]),

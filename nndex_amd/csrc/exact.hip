@@ -67,8 +67,8 @@ __global__ __launch_bounds__(256) void to_tiled_kernel(const float *__restrict__
 //   dims <  16: sequential f32 `sq += v*v` without fusion.
 // `get(c4)` returns float4 chunk c4 of the row.
 template <typename Get>
-__device__ inline float norm_sq_ref(Get get, uint32_t dims, bool &nonfinite) {
-    if (dims >= 16) {
+__device__ inline float norm_sq_ref(Get get, uint32_t simd_dims, uint32_t dims, bool &nonfinite) {
+    if (simd_dims >= 16) {
         float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
         uint32_t full = dims / 4;
         for (uint32_t c = 0; c < full; ++c) {
@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void normalize_tiled_kernel(float4 *__restrict
     if (row < rows) {
         float4 *base = x + (size_t)(row / 64) * d4 * 64 + (row % 64);
         bool nonfinite = false;
-        float sq = norm_sq_ref([&](uint32_t c) { return base[(size_t)c * 64]; }, dims, nonfinite);
+        float sq = norm_sq_ref([&](uint32_t c) { return base[(size_t)c * 64]; }, dims, dims,
+                               nonfinite);
         if (nonfinite) {
             f = 1;
         } else if (sq <= 1.1920929e-7f) {
@@ -136,12 +137,18 @@ __global__ __launch_bounds__(256) void normalize_tiled_kernel(float4 *__restrict
     if (__lane_id() == 0 && (any1 || any2)) atomicOr(flags, (any1 ? 1u : 0u) | (any2 ? 2u : 0u));
 }
 
-// Row-major normalisation (queries): src [rows][dims] -> dst [rows][pdims].
-// normalize != 0 applies normalize_rows (src/lib.rs:461-514); otherwise only
-// the finite check and zero padding of pad_rows (src/lib.rs:578-588).
+// Row-major query preparation: src [rows][dims] -> dst [rows][pdims].
+//   mode 0 (kQueryPad):    ensure_finite + pad_rows / pad_vector
+//                          (src/lib.rs:383-385, :405-408, :578-599)
+//   mode 1 (kQueryBatch):  normalize_rows (src/lib.rs:461-553): the simsimd
+//                          four-lane sum for dims >= 16, sequential below
+//   mode 2 (kQuerySingle): normalize_vector (src/lib.rs:558-575): always the
+//                          sequential unfused f32 sum of squares
+// A row that fails (bit 0 non-finite, bit 1 zero norm) is written as zeros;
+// the bits go to *flags (OR over rows) and, when given, row_flags[row].
 __global__ __launch_bounds__(256) void normalize_rowmajor_kernel(
-    const float *__restrict__ src, uint32_t rows, uint32_t dims, uint32_t pdims,
-    int normalize, float *__restrict__ dst, uint32_t *__restrict__ flags) {
+    const float *__restrict__ src, uint32_t rows, uint32_t dims, uint32_t pdims, int mode,
+    float *__restrict__ dst, uint32_t *__restrict__ flags, uint8_t *__restrict__ row_flags) {
     const uint32_t row = blockIdx.x * 256 + threadIdx.x;
     uint32_t f = 0;
     if (row < rows) {
@@ -152,9 +159,10 @@ __global__ __launch_bounds__(256) void normalize_rowmajor_kernel(
             for (uint32_t j = 0; j < 4; ++j) e[j] = c * 4 + j < dims ? s[c * 4 + j] : 0.f;
             return make_float4(e[0], e[1], e[2], e[3]);
         };
-        if (normalize) {
+        if (mode != kQueryPad) {
             bool nonfinite = false;
-            float sq = norm_sq_ref(get, dims, nonfinite);
+            // the single-vector form never takes the four-lane sum
+            const float sq = norm_sq_ref(get, mode == kQuerySingle ? 0u : dims, dims, nonfinite);
             if (nonfinite) {
                 f = 1;
             } else if (sq <= 1.1920929e-7f) {
@@ -170,7 +178,10 @@ __global__ __launch_bounds__(256) void normalize_rowmajor_kernel(
                 d[i] = v;
             }
         }
+        if (f)
+            for (uint32_t i = 0; i < dims; ++i) d[i] = 0.f;
         for (uint32_t i = dims; i < pdims; ++i) d[i] = 0.f;
+        if (row_flags) row_flags[row] = static_cast<uint8_t>(f);
     }
     uint32_t any1 = __ballot(f & 1) != 0, any2 = __ballot(f & 2) != 0;
     if (__lane_id() == 0 && (any1 || any2)) atomicOr(flags, (any1 ? 1u : 0u) | (any2 ? 2u : 0u));
@@ -445,9 +456,11 @@ void launch_normalize_tiled(float4 *x, uint32_t rows, uint32_t dims, uint32_t d4
 }
 
 void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, uint32_t pdims,
-                               bool normalize, float *dst, uint32_t *flags, hipStream_t s) {
+                               int mode, float *dst, uint32_t *flags, uint8_t *row_flags,
+                               hipStream_t s) {
+    if (rows == 0) return;
     hipLaunchKernelGGL(normalize_rowmajor_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, src,
-                       rows, dims, pdims, normalize ? 1 : 0, dst, flags);
+                       rows, dims, pdims, mode, dst, flags, row_flags);
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 

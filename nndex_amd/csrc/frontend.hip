@@ -2,10 +2,12 @@
 //
 // Kept from the reference: shape/k/dims validation and its error order
 // (src/lib.rs:185-402, 433-450), padding to a multiple of 8 (:192), k capping
-// (:347, :390), single-query preparation on the host (normalize_vector /
-// pad_vector, :404-411, :558-599), and the 64-entry constructor cache keyed
-// by source pointer, shape, options and a sampled fingerprint (:601-768).
-// Moved to the device: row normalisation of the matrix and of batch queries.
+// (:347, :390), and the 64-entry constructor cache keyed by source pointer,
+// shape, options and a sampled fingerprint (:601-768).
+// Moved to the device: row normalisation of the matrix and of batch queries,
+// and single-query preparation (normalize_vector / pad_vector, :404-411,
+// :558-599, in the reference's sequential order), with the query checks read
+// back together with the results.
 // This build carries the GPU backend only, exactly like the reference built
 // with `--features gpu` and without `cpu`: BackendPreference::Cpu yields
 // NNdexError::CpuBackendUnavailable (src/lib.rs:244-254).
@@ -44,30 +46,6 @@ Error gpu_error(const Error &e) {
     if (e.code >= NNDEX_ERR_GPU_ADAPTER_NOT_FOUND && e.code <= NNDEX_ERR_GPU_DEVICE_INDEX_OUT_OF_RANGE)
         return Error(e.code, std::string("gpu execution error: ") + e.what());
     return e;
-}
-
-// normalize_vector, src/lib.rs:558-575: sequential f32 sum of squares, no fusion.
-std::vector<float> normalize_vector(const float *v, size_t dims, size_t pdims) {
-    float norm_sq = 0.f;
-    for (size_t i = 0; i < dims; ++i) {
-        if (!std::isfinite(v[i])) throw Error(NNDEX_ERR_NON_FINITE_INPUT, "all input values must be finite");
-        norm_sq += v[i] * v[i];  // unfused: the library builds with -ffp-contract=off
-    }
-    if (norm_sq <= kF32Epsilon) throw Error(NNDEX_ERR_ZERO_NORM, "encountered a zero-norm vector");
-    float inv = 1.0f / std::sqrt(norm_sq);
-    std::vector<float> out(pdims, 0.f);
-    for (size_t i = 0; i < dims; ++i) out[i] = v[i] * inv;
-    return out;
-}
-
-// ensure_finite + pad_vector, src/lib.rs:405-408, 591-599.
-std::vector<float> pad_vector(const float *v, size_t dims, size_t pdims) {
-    std::vector<float> out(pdims, 0.f);
-    for (size_t i = 0; i < dims; ++i) {
-        if (!std::isfinite(v[i])) throw Error(NNDEX_ERR_NON_FINITE_INPUT, "all input values must be finite");
-        out[i] = v[i];
-    }
-    return out;
 }
 
 // Host restatement of the row checks of normalize_rows (src/lib.rs:519-553),
@@ -266,12 +244,10 @@ nndex_status nndex_search(nndex_index *index, const float *query, size_t query_l
             throw Error(NNDEX_ERR_QUERY_DIM_MISMATCH,
                         "query dimensionality mismatch: expected " + std::to_string(f.dims) +
                             ", found " + std::to_string(query_len));
-        std::vector<float> q = f.normalized ? pad_vector(query, f.dims, f.pdims)
-                                            : normalize_vector(query, f.dims, f.pdims);
         const size_t capped = k < f.rows ? k : f.rows;
         std::vector<Neighbor> res;
         try {
-            res = f.gpu->search(q.data(), capped, f.approx);
+            res = f.gpu->search_raw(query, f.dims, !f.normalized, capped, f.approx);
         } catch (const Error &e) {
             throw gpu_error(e);
         }

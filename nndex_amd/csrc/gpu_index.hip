@@ -17,8 +17,8 @@ void clear_last_error() { g_last_error.clear(); }
 // Result caches of src/gpu.rs:21-25: cleared wholesale when full.
 constexpr size_t kSingleCacheCapacity = 128;
 constexpr size_t kBatchCacheCapacity = 16;
-// Rows uploaded per staging chunk during construction.
-constexpr size_t kUploadChunkRows = 1u << 16;
+// Bytes per host -> device staging chunk during construction.
+constexpr size_t kUploadChunkBytes = size_t(16) << 20;
 
 void DeviceBuffer::reserve(size_t n) {
     if (n <= bytes) return;
@@ -243,34 +243,62 @@ void GpuIndex::upload_tiled(const float *host, size_t rows, size_t src_dims, boo
                             bool normalize, size_t logical_dims) {
     const size_t tiles = padded_rows(rows) / kTileRows;
     NNDEX_HIP_CHECK(hipMalloc(&x_, tiles * d4_ * kTileRows * sizeof(float4)));
-    DeviceBuffer staging, flags;
-    const size_t chunk = rows < kUploadChunkRows ? rows : kUploadChunkRows;
-    staging.reserve(chunk * src_dims * sizeof(float));
-    flags.reserve(sizeof(uint32_t));
-    NNDEX_HIP_CHECK(hipMemsetAsync(flags.ptr, 0, sizeof(uint32_t), stream_));
-    for (size_t r0 = 0; r0 < rows; r0 += chunk) {
-        size_t n = rows - r0 < chunk ? rows - r0 : chunk;
-        NNDEX_HIP_CHECK(hipMemcpyAsync(staging.ptr, host + r0 * src_dims,
-                                       n * src_dims * sizeof(float), hipMemcpyHostToDevice,
-                                       stream_));
-        launch_to_tiled(static_cast<const float *>(staging.ptr), static_cast<uint32_t>(n),
-                        static_cast<uint32_t>(src_dims), static_cast<uint32_t>(d4_),
-                        x_ + (r0 / kTileRows) * d4_ * kTileRows,
-                        static_cast<uint32_t *>(flags.ptr), stream_);
+    // Rows go host -> pinned staging -> device in chunks of about 16 MiB (a
+    // multiple of 64 rows): the host copy of chunk i + 1 (from a numpy array
+    // or a memory-mapped file) overlaps the DMA and tiling of chunk i, two
+    // pinned and two device buffers in turn, an event guarding each reuse.
+    size_t chunk = (kUploadChunkBytes / (src_dims * sizeof(float))) / kTileRows * kTileRows;
+    if (chunk < kTileRows) chunk = kTileRows;
+    if (chunk > rows) chunk = rows;
+    DeviceBuffer staging[2], flags;
+    PinnedBuffer pinned[2];
+    hipEvent_t done[2] = {nullptr, nullptr};
+    auto release = [&] {
+        for (hipEvent_t e : done)
+            if (e) (void)hipEventDestroy(e);
+    };
+    try {
+        for (int b = 0; b < 2; ++b) {
+            staging[b].reserve(chunk * src_dims * sizeof(float));
+            pinned[b].reserve(chunk * src_dims * sizeof(float));
+            NNDEX_HIP_CHECK(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+        }
+        flags.reserve(sizeof(uint32_t));
+        launch_zero_u32(static_cast<uint32_t *>(flags.ptr), 1, stream_);
+        size_t i = 0;
+        for (size_t r0 = 0; r0 < rows; r0 += chunk, ++i) {
+            const int b = static_cast<int>(i & 1);
+            const size_t n = rows - r0 < chunk ? rows - r0 : chunk;
+            const size_t bytes = n * src_dims * sizeof(float);
+            if (i >= 2) NNDEX_HIP_CHECK(hipEventSynchronize(done[b]));  // chunk i - 2 is done
+            std::memcpy(pinned[b].ptr, host + r0 * src_dims, bytes);
+            NNDEX_HIP_CHECK(hipMemcpyAsync(staging[b].ptr, pinned[b].ptr, bytes,
+                                           hipMemcpyHostToDevice, stream_));
+            launch_to_tiled(static_cast<const float *>(staging[b].ptr), static_cast<uint32_t>(n),
+                            static_cast<uint32_t>(src_dims), static_cast<uint32_t>(d4_),
+                            x_ + (r0 / kTileRows) * d4_ * kTileRows,
+                            static_cast<uint32_t *>(flags.ptr), stream_);
+            NNDEX_HIP_CHECK(hipEventRecord(done[b], stream_));
+        }
+        if (normalize)
+            launch_normalize_tiled(x_, static_cast<uint32_t>(rows),
+                                   static_cast<uint32_t>(logical_dims), static_cast<uint32_t>(d4_),
+                                   static_cast<uint32_t *>(flags.ptr), stream_);
+        NNDEX_HIP_CHECK(hipMalloc(&xmax_sq_, sizeof(unsigned long long)));
+        launch_zero_u32(reinterpret_cast<uint32_t *>(xmax_sq_), 2, stream_);
+        launch_max_norm(x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(d4_), xmax_sq_,
+                        stream_);
+        uint32_t *h_flags = static_cast<uint32_t *>(pinned[0].ptr);
+        NNDEX_HIP_CHECK(hipMemcpyAsync(h_flags, flags.ptr, sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, stream_));
+        NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
+        release();
+        if (check_finite || normalize) throw_flags(*h_flags);
+    } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        release();
+        throw;
     }
-    if (normalize)
-        launch_normalize_tiled(x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(logical_dims),
-                               static_cast<uint32_t>(d4_), static_cast<uint32_t *>(flags.ptr),
-                               stream_);
-    NNDEX_HIP_CHECK(hipMalloc(&xmax_sq_, sizeof(unsigned long long)));
-    NNDEX_HIP_CHECK(hipMemsetAsync(xmax_sq_, 0, sizeof(unsigned long long), stream_));
-    launch_max_norm(x_, static_cast<uint32_t>(rows), static_cast<uint32_t>(d4_), xmax_sq_,
-                    stream_);
-    uint32_t h_flags = 0;
-    NNDEX_HIP_CHECK(hipMemcpyAsync(&h_flags, flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                   stream_));
-    NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
-    if (check_finite || normalize) throw_flags(h_flags);
 }
 
 void GpuIndex::ensure_bf16(hipStream_t s) {
@@ -977,7 +1005,7 @@ uint64_t GpuIndex::query_hash(const float *values, size_t n, size_t k, size_t qu
 }
 
 std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size_t query_rows,
-                                                      size_t k, size_t raw_dims, bool normalize,
+                                                      size_t k, size_t raw_dims, int mode,
                                                       bool approx) {
     NNDEX_HIP_CHECK(hipSetDevice(device_));
     // dispatch_*_or_approx (src/gpu.rs:359-367, :404-432)
@@ -993,22 +1021,25 @@ std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size
     h_idx_.reserve(obytes * sizeof(uint64_t));
     h_sim_.reserve(obytes * sizeof(float));
     std::memcpy(h_q_.ptr, queries, in_bytes);
+    // raw queries are checked and prepared on the device; their flags come
+    // back with the results (one synchronisation for the whole call), and a
+    // failed row was zeroed, so the search over it is well defined
+    uint32_t *h_flags = nullptr;
     if (raw_dims) {
         d_raw_.reserve(in_bytes);
         d_flags_.reserve(sizeof(uint32_t));
-        NNDEX_HIP_CHECK(hipMemsetAsync(d_flags_.ptr, 0, sizeof(uint32_t), stream_));
+        h_flag_.reserve(sizeof(uint32_t));
+        h_flags = static_cast<uint32_t *>(h_flag_.ptr);
+        launch_zero_u32(static_cast<uint32_t *>(d_flags_.ptr), 1, stream_);
         NNDEX_HIP_CHECK(
             hipMemcpyAsync(d_raw_.ptr, h_q_.ptr, in_bytes, hipMemcpyHostToDevice, stream_));
         launch_normalize_rowmajor(static_cast<const float *>(d_raw_.ptr),
                                   static_cast<uint32_t>(query_rows), static_cast<uint32_t>(raw_dims),
-                                  static_cast<uint32_t>(pdims_), normalize,
+                                  static_cast<uint32_t>(pdims_), mode,
                                   static_cast<float *>(d_q_.ptr),
-                                  static_cast<uint32_t *>(d_flags_.ptr), stream_);
-        uint32_t h_flags = 0;
-        NNDEX_HIP_CHECK(hipMemcpyAsync(&h_flags, d_flags_.ptr, sizeof(uint32_t),
+                                  static_cast<uint32_t *>(d_flags_.ptr), nullptr, stream_);
+        NNDEX_HIP_CHECK(hipMemcpyAsync(h_flags, d_flags_.ptr, sizeof(uint32_t),
                                        hipMemcpyDeviceToHost, stream_));
-        NNDEX_HIP_CHECK(hipStreamSynchronize(stream_));
-        throw_flags(h_flags);
     } else {
         NNDEX_HIP_CHECK(
             hipMemcpyAsync(d_q_.ptr, h_q_.ptr, qbytes, hipMemcpyHostToDevice, stream_));
@@ -1033,6 +1064,7 @@ std::vector<std::vector<Neighbor>> GpuIndex::run_host(const float *queries, size
                                    hipMemcpyDeviceToHost, stream_));
     if (hipStreamSynchronize(stream_) != hipSuccess)
         throw Error(NNDEX_ERR_GPU_MAP_READ_FAILED, "failed to map GPU buffer for reading");
+    if (h_flags) throw_flags(*h_flags);
     std::vector<std::vector<Neighbor>> out(query_rows);
     const uint64_t *hi = static_cast<const uint64_t *>(h_idx_.ptr);
     const float *hs = static_cast<const float *>(h_sim_.ptr);
@@ -1051,12 +1083,12 @@ std::vector<Neighbor> GpuIndex::search(const float *query, size_t k, bool approx
         uint64_t key = query_hash(query, pdims_, k, 1, approx);
         auto it = cache_single_.find(key);
         if (it != cache_single_.end()) return it->second;
-        auto res = std::move(run_host(query, 1, capped, 0, false, approx)[0]);
+        auto res = std::move(run_host(query, 1, capped, 0, kQueryPad, approx)[0]);
         if (cache_single_.size() >= kSingleCacheCapacity) cache_single_.clear();
         cache_single_.emplace(key, res);
         return res;
     }
-    return std::move(run_host(query, 1, capped, 0, false, approx)[0]);
+    return std::move(run_host(query, 1, capped, 0, kQueryPad, approx)[0]);
 }
 
 std::vector<std::vector<Neighbor>> GpuIndex::search_batch(const float *queries, size_t query_rows,
@@ -1068,12 +1100,12 @@ std::vector<std::vector<Neighbor>> GpuIndex::search_batch(const float *queries, 
         uint64_t key = query_hash(queries, query_rows * pdims_, k, query_rows, approx);
         auto it = cache_batch_.find(key);
         if (it != cache_batch_.end()) return it->second;
-        auto res = run_host(queries, query_rows, capped, 0, false, approx);
+        auto res = run_host(queries, query_rows, capped, 0, kQueryPad, approx);
         if (cache_batch_.size() >= kBatchCacheCapacity) cache_batch_.clear();
         cache_batch_.emplace(key, res);
         return res;
     }
-    return run_host(queries, query_rows, capped, 0, false, approx);
+    return run_host(queries, query_rows, capped, 0, kQueryPad, approx);
 }
 
 std::vector<std::vector<Neighbor>> GpuIndex::search_batch_raw(const float *queries,
@@ -1090,12 +1122,34 @@ std::vector<std::vector<Neighbor>> GpuIndex::search_batch_raw(const float *queri
                        (normalize ? 0x9e3779b97f4a7c15ull : 0ull);
         auto it = cache_batch_.find(key);
         if (it != cache_batch_.end()) return it->second;
-        auto res = run_host(queries, query_rows, capped, dims, normalize, approx);
+        auto res = run_host(queries, query_rows, capped, dims,
+                            normalize ? kQueryBatch : kQueryPad, approx);
         if (cache_batch_.size() >= kBatchCacheCapacity) cache_batch_.clear();
         cache_batch_.emplace(key, res);
         return res;
     }
-    return run_host(queries, query_rows, capped, dims, normalize, approx);
+    return run_host(queries, query_rows, capped, dims, normalize ? kQueryBatch : kQueryPad,
+                    approx);
+}
+
+std::vector<Neighbor> GpuIndex::search_raw(const float *query, size_t dims, bool normalize,
+                                           size_t k, bool approx) {
+    std::lock_guard<std::mutex> lock(mu_);
+    const size_t capped = k < rows_ ? k : rows_;
+    const int mode = normalize ? kQuerySingle : kQueryPad;
+    if (enable_cache_) {
+        // prepare_query is deterministic: a key over the raw values and the
+        // mode hits exactly when one over the prepared query would
+        uint64_t key = query_hash(query, dims, k, 1, approx) ^
+                       (normalize ? 0x5bd1e9955bd1e995ull : 0x27d4eb2f165667c5ull);
+        auto it = cache_single_.find(key);
+        if (it != cache_single_.end()) return it->second;
+        auto res = std::move(run_host(query, 1, capped, dims, mode, approx)[0]);
+        if (cache_single_.size() >= kSingleCacheCapacity) cache_single_.clear();
+        cache_single_.emplace(key, res);
+        return res;
+    }
+    return std::move(run_host(query, 1, capped, dims, mode, approx)[0]);
 }
 
 // ---------------------------------------------------------------------------

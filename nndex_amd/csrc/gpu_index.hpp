@@ -191,6 +191,11 @@ class GpuIndex {
     std::vector<std::vector<Neighbor>> search_batch_raw(const float *queries, size_t query_rows,
                                                         size_t dims, bool normalize, size_t k,
                                                         bool approx);
+    // Front-end single query (src/lib.rs:335-355): `dims` raw values checked
+    // and padded, or normalised in normalize_vector's sequential order
+    // (src/lib.rs:558-575), on the device, then searched.
+    std::vector<Neighbor> search_raw(const float *query, size_t dims, bool normalize, size_t k,
+                                     bool approx);
 
     // Enqueue an exact search of device-resident queries.  With `heap_ties`,
     // rows tied at the k-th score are chosen as the reference GPU path's
@@ -300,10 +305,10 @@ class GpuIndex {
                       bool normalize, size_t logical_dims);
     uint64_t query_hash(const float *values, size_t n, size_t k, size_t query_rows,
                         bool approx) const;
-    // raw_dims == 0: queries are normalised and padded already.
+    // raw_dims == 0: queries are normalised and padded already; else they
+    // are prepared on the device in `mode` (kQuery*).
     std::vector<std::vector<Neighbor>> run_host(const float *queries, size_t query_rows,
-                                                size_t k, size_t raw_dims, bool normalize,
-                                                bool approx);
+                                                size_t k, size_t raw_dims, int mode, bool approx);
 
     int device_ = 0;
     uint32_t ncu_ = 0;
@@ -334,7 +339,7 @@ class GpuIndex {
     std::unique_ptr<IvfIndex> ivf_;
     size_t ivf_nlist_ = 0, ivf_nprobe_ = 0, ivf_iters_ = 0;
     DeviceBuffer d_q_, d_raw_, d_idx_, d_sim_, d_flags_, d_len_;
-    PinnedBuffer h_q_, h_idx_, h_sim_;
+    PinnedBuffer h_q_, h_idx_, h_sim_, h_flag_;
     std::unordered_map<uint64_t, std::vector<Neighbor>> cache_single_;
     std::unordered_map<uint64_t, std::vector<std::vector<Neighbor>>> cache_batch_;
 };

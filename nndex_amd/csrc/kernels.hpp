@@ -38,8 +38,15 @@ void launch_to_tiled(const float *src, uint32_t rows, uint32_t dims, uint32_t d4
                      uint32_t *flags, hipStream_t s);
 void launch_normalize_tiled(float4 *x, uint32_t rows, uint32_t dims, uint32_t d4,
                             uint32_t *flags, hipStream_t s);
+// Query preparation modes of launch_normalize_rowmajor.
+constexpr int kQueryPad = 0;     // ensure_finite + pad (normalized = true)
+constexpr int kQueryBatch = 1;   // normalize_rows (batch queries)
+constexpr int kQuerySingle = 2;  // normalize_vector (a single query)
+// Rows failing the checks are zeroed; flag bits (1 non-finite, 2 zero norm)
+// are ORed into *flags and, if row_flags is given, stored per row.
 void launch_normalize_rowmajor(const float *src, uint32_t rows, uint32_t dims, uint32_t pdims,
-                               bool normalize, float *dst, uint32_t *flags, hipStream_t s);
+                               int mode, float *dst, uint32_t *flags, uint8_t *row_flags,
+                               hipStream_t s);
 
 // Slots per wave for a given k (power of two >= k + 64); 0 QB means k is too
 // large for the fused scan.

@@ -53,7 +53,11 @@ struct Request {
     uint32_t nprobe = 0;   // IVF: resolved probe count
     uint32_t k = 0;        // results per row (capped at rows)
     uint32_t nq = 0;
-    std::vector<float> q;  // nq x pdims
+    // raw requests (raw_dims > 0): rows of raw_dims values, checked and
+    // prepared on the device in raw_mode (kQueryPad / kQueryBatch)
+    uint32_t raw_dims = 0;
+    int raw_mode = 0;
+    std::vector<float> q;  // nq x (raw_dims ? raw_dims : pdims)
     std::vector<uint64_t> idx, lens;
     std::vector<float> sim;
     uint32_t issued = 0, done = 0;
@@ -66,13 +70,14 @@ struct Request {
 struct Segment {
     std::shared_ptr<Request> req;
     uint32_t q0, n;
+    uint32_t row_flags = 0;  // raw rows: OR of the device checks (1 non-finite, 2 zero norm)
 };
 
 struct Executor {
     hipStream_t stream = nullptr;
     std::unique_ptr<Workspace> ws;
-    DeviceBuffer dq, didx, dsim, dlen, dflags;
-    PinnedBuffer hq, hidx, hsim, hlen, hflags;
+    DeviceBuffer dq, didx, dsim, dlen, dflags, draw, drowflags, dflagword;
+    PinnedBuffer hq, hidx, hsim, hlen, hflags, hrowflags;
     // captured exact batches by (rows bucket, k bucket), made on first use
     std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<BatchGraph>> graphs;
 };
@@ -81,7 +86,11 @@ class Server {
   public:
     Server(std::shared_ptr<GpuIndex> index, const nndex_server_options &o);
     ~Server();
-    uint64_t submit(const float *q, size_t nq, size_t k, int mode, size_t nprobe);
+    // raw_dims == 0: normalised padded rows; else rows of raw_dims values
+    // prepared on the device (normalize: normalize_rows, else ensure_finite +
+    // pad), a failed check reported by wait for that request
+    uint64_t submit(const float *q, size_t nq, size_t k, int mode, size_t nprobe,
+                    size_t raw_dims = 0, bool normalize = false);
     void wait(uint64_t ticket, uint64_t *idx, float *sim, uint64_t *lens, double *latency_ms);
     void stats(nndex_server_stats &out);
     void reset_stats();
@@ -125,7 +134,8 @@ Server::Server(std::shared_ptr<GpuIndex> index, const nndex_server_options &o)
     const size_t np = opt_.ivf_nprobe_max;
     NNDEX_HIP_CHECK(hipSetDevice(index_->device()));
     // footprint of one executor: workspace + device query/result buffers
-    slot_bytes_ = index_->workspace_bytes(B, K, np) + B * pdims_ * 4 + B * K * 12 + B * 9;
+    slot_bytes_ = index_->workspace_bytes(B, K, np) + 2 * B * pdims_ * 4 + B * K * 12 + B * 10 +
+                  256;
     size_t slots = opt_.hbm_budget_bytes / slot_bytes_;
     if (slots > opt_.slots) slots = opt_.slots;
     if (slots == 0)
@@ -145,6 +155,10 @@ Server::Server(std::shared_ptr<GpuIndex> index, const nndex_server_options &o)
         ex->dlen.reserve(B * 8);
         ex->dflags.reserve(B);
         ex->hflags.reserve(B);
+        ex->draw.reserve(B * pdims_ * 4);
+        ex->drowflags.reserve(B);
+        ex->dflagword.reserve(sizeof(uint32_t));
+        ex->hrowflags.reserve(B);
         ex->hq.reserve(B * pdims_ * 4);
         ex->hidx.reserve(B * K * 8);
         ex->hsim.reserve(B * K * 4);
@@ -168,8 +182,13 @@ Server::~Server() {
         }
 }
 
-uint64_t Server::submit(const float *q, size_t nq, size_t k, int mode, size_t nprobe) {
+uint64_t Server::submit(const float *q, size_t nq, size_t k, int mode, size_t nprobe,
+                        size_t raw_dims, bool normalize) {
     if (nq == 0) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "a request needs at least one row");
+    if (raw_dims && (raw_dims + 7) / 8 * 8 != pdims_)
+        throw Error(NNDEX_ERR_QUERY_DIM_MISMATCH,
+                    "query dimensionality mismatch: rows of " + std::to_string(raw_dims) +
+                        " values do not pad to the index's " + std::to_string(pdims_));
     if (k == 0) throw Error(NNDEX_ERR_INVALID_TOP_K, "top-k must be greater than zero");
     if (mode != 0 && mode != 1) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "mode must be 0 or 1");
     const size_t kc = k < rows_ ? k : rows_;
@@ -194,7 +213,9 @@ uint64_t Server::submit(const float *q, size_t nq, size_t k, int mode, size_t np
                             std::to_string(opt_.ivf_nprobe_max) + ")");
         r->nprobe = static_cast<uint32_t>(np);
     }
-    r->q.assign(q, q + nq * pdims_);
+    r->raw_dims = static_cast<uint32_t>(raw_dims);
+    r->raw_mode = normalize ? kQueryBatch : kQueryPad;
+    r->q.assign(q, q + nq * (raw_dims ? raw_dims : pdims_));
     r->idx.assign(nq * kc, 0);
     r->sim.assign(nq * kc, 0.f);
     r->lens.assign(nq, 0);
@@ -219,10 +240,13 @@ bool Server::next_batch(std::vector<Segment> &batch) {
     if (pending_.empty()) return false;
     const int mode = pending_.front()->mode;
     const uint32_t nprobe = pending_.front()->nprobe;
+    const uint32_t raw_dims = pending_.front()->raw_dims;
+    const int raw_mode = pending_.front()->raw_mode;
     uint32_t rows = 0;
     for (auto it = pending_.begin(); it != pending_.end() && rows < opt_.max_batch_queries;) {
         Request &r = **it;
-        if (r.mode != mode || r.nprobe != nprobe || r.issued == r.nq) {
+        if (r.mode != mode || r.nprobe != nprobe || r.raw_dims != raw_dims ||
+            (raw_dims && r.raw_mode != raw_mode) || r.issued == r.nq) {
             ++it;
             continue;
         }
@@ -246,13 +270,34 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
     }
     const int mode = batch[0].req->mode;
     const uint32_t nprobe = batch[0].req->nprobe;
+    const uint32_t raw_dims = batch[0].req->raw_dims;
+    const int raw_mode = batch[0].req->raw_mode;
+    const size_t cols = raw_dims ? raw_dims : pdims_;  // staged values per row
     float *hq = static_cast<float *>(ex.hq.ptr);
     size_t off = 0;
     for (auto &sg : batch) {
-        std::memcpy(hq + off * pdims_, sg.req->q.data() + (size_t)sg.q0 * pdims_,
-                    (size_t)sg.n * pdims_ * sizeof(float));
+        std::memcpy(hq + off * cols, sg.req->q.data() + (size_t)sg.q0 * cols,
+                    (size_t)sg.n * cols * sizeof(float));
         off += sg.n;
     }
+    // rows to the device: as they are, or raw rows checked and prepared into
+    // dq there (their per-row flags come back with the results)
+    auto upload = [&](uint32_t rows) {
+        if (!raw_dims) {
+            NNDEX_HIP_CHECK(hipMemcpyAsync(ex.dq.ptr, hq, (size_t)rows * pdims_ * 4,
+                                           hipMemcpyHostToDevice, ex.stream));
+            return;
+        }
+        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.draw.ptr, hq, (size_t)rows * cols * 4,
+                                       hipMemcpyHostToDevice, ex.stream));
+        launch_normalize_rowmajor(static_cast<const float *>(ex.draw.ptr), rows, raw_dims,
+                                  static_cast<uint32_t>(pdims_), raw_mode,
+                                  static_cast<float *>(ex.dq.ptr),
+                                  static_cast<uint32_t *>(ex.dflagword.ptr),
+                                  static_cast<uint8_t *>(ex.drowflags.ptr), ex.stream);
+        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hrowflags.ptr, ex.drowflags.ptr, rows,
+                                       hipMemcpyDeviceToHost, ex.stream));
+    };
     const float *dq = static_cast<const float *>(ex.dq.ptr);
     uint64_t *didx = static_cast<uint64_t *>(ex.didx.ptr);
     float *dsim = static_cast<float *>(ex.dsim.ptr);
@@ -269,9 +314,8 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
         while (kb < k) kb <<= 1;
         if (kb > opt_.max_k) kb = opt_.max_k;
         for (uint32_t r = nq; r < qb; ++r)
-            std::memcpy(hq + (size_t)r * pdims_, hq + (size_t)(nq - 1) * pdims_, pdims_ * 4);
-        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.dq.ptr, hq, (size_t)qb * pdims_ * 4,
-                                       hipMemcpyHostToDevice, ex.stream));
+            std::memcpy(hq + (size_t)r * cols, hq + (size_t)(nq - 1) * cols, cols * 4);
+        upload(qb);
         auto &g = ex.graphs[{qb, kb}];
         uint8_t *dflags = static_cast<uint8_t *>(ex.dflags.ptr);
         // (re)capture when new, or when a workspace buffer moved since
@@ -291,8 +335,7 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
         kstride = kb;
         graphed = true;
     } else {
-        NNDEX_HIP_CHECK(hipMemcpyAsync(ex.dq.ptr, hq, (size_t)nq * pdims_ * 4,
-                                       hipMemcpyHostToDevice, ex.stream));
+        upload(nq);
         if (mode == 0)
             index_->search_exact_any(*ex.ws, dq, nq, k, didx, dsim, ex.stream);
         else
@@ -344,6 +387,16 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
             NNDEX_HIP_CHECK(hipMemcpyAsync(ex.hlen.ptr, ex.dlen.ptr, (size_t)nq * 8,
                                            hipMemcpyDeviceToHost, ex.stream));
         if (again) NNDEX_HIP_CHECK(hipStreamSynchronize(ex.stream));
+    }
+    // raw rows that failed their checks fail their own request only (every
+    // row's search is independent of its batch mates)
+    if (raw_dims) {
+        const uint8_t *rf = static_cast<const uint8_t *>(ex.hrowflags.ptr);
+        off = 0;
+        for (auto &sg : batch) {
+            for (uint32_t i = 0; i < sg.n; ++i) sg.row_flags |= rf[off + i];
+            off += sg.n;
+        }
     }
     // scatter: each request keeps the first k of its rows' results
     off = 0;
@@ -399,6 +452,14 @@ void Server::run(size_t e) {
                 if (st != NNDEX_OK && r.status == NNDEX_OK) {
                     r.status = st;
                     r.error = err;
+                }
+                if (st == NNDEX_OK && sg.row_flags && r.status == NNDEX_OK) {
+                    try {
+                        throw_flags(sg.row_flags);
+                    } catch (const Error &x) {
+                        r.status = x.code;
+                        r.error = x.what();
+                    }
                 }
                 r.done += sg.n;
                 if (r.done == r.nq) {
@@ -510,6 +571,16 @@ nndex_status nndex_server_submit(nndex_server *server, const float *queries, siz
     return guard([&] {
         if (!server || !queries || !ticket) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
         *ticket = server->impl->submit(queries, query_rows, k, mode, nprobe);
+    });
+}
+
+nndex_status nndex_server_submit_raw(nndex_server *server, const float *queries,
+                                     size_t query_rows, size_t dims, int32_t normalize, size_t k,
+                                     int32_t mode, size_t nprobe, uint64_t *ticket) {
+    return guard([&] {
+        if (!server || !queries || !ticket) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (dims == 0) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "rows need at least one value");
+        *ticket = server->impl->submit(queries, query_rows, k, mode, nprobe, dims, normalize != 0);
     });
 }
 

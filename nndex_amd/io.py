@@ -4,11 +4,18 @@
 list<f32> or fixed_size_list<f32> column named ``key``) become a row-major
 float32 matrix.  Error texts follow DiskLoadError (src/python_io.rs:20-74).
 Arrays are read with ``allow_pickle=False``.
+
+A ``.npy`` file, and an ``.npz`` member stored without compression (what
+``numpy.savez`` writes), are memory-mapped rather than read: construction then
+copies the rows from the file's pages into pinned staging buffers and on to
+the device in chunks, and no host copy of the whole matrix is made (the
+reference reads the file into a Vec first, src/python_io.rs:112-165).
 """
 
 from __future__ import annotations
 
 import os
+import zipfile
 
 import numpy as np
 
@@ -35,8 +42,8 @@ def load_matrix_from_disk(path: str, key: str | None = None) -> np.ndarray:
     ext = os.path.splitext(path)[1].lower().lstrip(".")
     if ext == "npy":
         try:
-            arr = np.load(path, allow_pickle=False)
-        except OSError as e:
+            arr = np.load(path, mmap_mode="r", allow_pickle=False)
+        except (OSError, ValueError) as e:
             raise DiskLoadError(f"failed to read .npy matrix from '{path}': {e}") from None
         return _as_matrix(path, arr)
     if ext == "npz":
@@ -48,7 +55,9 @@ def load_matrix_from_disk(path: str, key: str | None = None) -> np.ndarray:
                 if key not in names:
                     raise DiskLoadError(f"key '{key}' not found in .npz '{path}'. "
                                         f"Available keys: {', '.join(names)}")
-                arr = z[key]
+                arr = _npz_member_mmap(path, key)
+                if arr is None:
+                    arr = z[key]
         except OSError as e:
             raise DiskLoadError(f"failed to read .npz archive from '{path}': {e}") from None
         return _as_matrix(path, arr)
@@ -58,6 +67,39 @@ def load_matrix_from_disk(path: str, key: str | None = None) -> np.ndarray:
         return _load_parquet(path, key)
     raise DiskLoadError(f"unsupported file extension for '{path}', expected .npy, .npz, "
                         "or .parquet")
+
+
+def _npz_member_mmap(path: str, key: str):
+    """The member ``key`` of an .npz archive as a read-only memory map when
+    it is stored without compression and in C order; None otherwise."""
+    with zipfile.ZipFile(path) as zf:
+        try:
+            info = zf.getinfo(key + ".npy")
+        except KeyError:
+            return None
+        if info.compress_type != zipfile.ZIP_STORED:
+            return None
+        with open(path, "rb") as f:
+            # local file header: 30 bytes, then the name and extra field
+            f.seek(info.header_offset)
+            local = f.read(30)
+            if local[:4] != b"PK\x03\x04":
+                return None
+            name_len = int.from_bytes(local[26:28], "little")
+            extra_len = int.from_bytes(local[28:30], "little")
+            start = info.header_offset + 30 + name_len + extra_len
+            f.seek(start)
+            version = np.lib.format.read_magic(f)
+            if version == (1, 0):
+                shape, fortran, dtype = np.lib.format.read_array_header_1_0(f)
+            elif version == (2, 0):
+                shape, fortran, dtype = np.lib.format.read_array_header_2_0(f)
+            else:
+                return None
+            data = f.tell()
+    if fortran or dtype.hasobject or len(shape) == 0:
+        return None
+    return np.memmap(path, dtype=dtype, mode="r", offset=data, shape=tuple(shape), order="C")
 
 
 def _load_parquet(path: str, key: str) -> np.ndarray:

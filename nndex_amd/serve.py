@@ -57,6 +57,21 @@ class Server:
         self._meta[t.value] = (q.shape[0], min(k, self.rows))
         return t.value
 
+    def submit_raw(self, queries: np.ndarray, k: int, mode: int = EXACT, nprobe: int = 0,
+                   normalize: bool = True) -> int:
+        """Raw rows (``dims`` columns, padding to the index's width) checked
+        and, with ``normalize``, normalised on the device as the front-end's
+        batch search does; a failed check fails this request at ``wait``."""
+        q = np.ascontiguousarray(queries, np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        t = ctypes.c_uint64()
+        _lib.check(self._L.nndex_server_submit_raw(self._h, q.ctypes.data_as(_FP), q.shape[0],
+                                                   q.shape[1], int(bool(normalize)), k, mode,
+                                                   nprobe, ctypes.byref(t)))
+        self._meta[t.value] = (q.shape[0], min(k, self.rows))
+        return t.value
+
     def wait(self, ticket: int):
         nq, k = self._meta.pop(ticket)
         idx = np.zeros((nq, k), np.uint64)
