@@ -89,6 +89,29 @@ class Dev:
         return ctypes.c_void_p(self.ptr.value + int(nbytes))
 
 
+def fp64_err(base, queries, idx, sim, sample) -> float:
+    """Largest |returned similarity - FP64 dot of that row and query| over the
+    sampled queries (numpy float64 on the returned rows only)."""
+    worst = 0.0
+    for i in sample:
+        rows = base[idx[i].astype(np.int64)].astype(np.float64)
+        ref = rows @ queries[i].astype(np.float64)
+        worst = max(worst, float(np.max(np.abs(sim[i].astype(np.float64) - ref))))
+    return worst
+
+
+def host_threads() -> tuple[int, int, str]:
+    """Threads for the CPU baseline: every core the process may run on,
+    capped by OMP_NUM_THREADS when the job sets it (the host's share for this
+    job); also nproc and the reason."""
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and 0 < int(omp) < avail:
+        return int(omp), nproc, f"OMP_NUM_THREADS={omp} of {avail} cores in the affinity mask"
+    return avail, nproc, f"all {avail} cores in the affinity mask"
+
+
 def event():
     e = ctypes.c_void_p()
     check(_lib.lib().nndex_event_create(ctypes.byref(e)))
@@ -161,13 +184,19 @@ def ivf_bench(args, k: int = 10) -> dict:
                                 for i in range(nq)]))
         probed = np.argpartition(-cscore, nprobe - 1, axis=1)[:, :nprobe]
         scanned = int(sizes[probed].sum())
-        gbs = scanned * dims * 4 / dt / 1e9
+        distinct = int(sizes[np.unique(probed)].sum())
+        # the per-query-equivalent rate: bytes of every (query, probed list)
+        # pair over the batch time; not HBM traffic (the list-major scan reads
+        # each probed list once per group of queries that probe it)
+        qeq = scanned * dims * 4 / dt / 1e9
         res[f"nprobe_{nprobe}"] = {"qps": round(nq / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
                                    f"recall_at_{k}": round(recall, 5),
                                    "rows_scanned_per_query": round(scanned / nq, 1),
-                                   "list_scan_gbs": round(gbs, 1)}
+                                   "per_query_equivalent_gbs": round(qeq, 1),
+                                   "distinct_list_bytes_per_batch": distinct * dims * 4}
         log(f"ivf nprobe {nprobe}: {nq / dt:.0f} QPS ({dt * 1e3:.2f} ms), recall@{k} "
-            f"{recall:.4f}, {scanned / nq:.0f} rows/query, {gbs:.0f} GB/s")
+            f"{recall:.4f}, {scanned / nq:.0f} rows/query, per-query-equivalent {qeq:.0f} GB/s, "
+            f"distinct lists {distinct * dims * 4 / 1e6:.1f} MB")
     L.nndex_stream_destroy(stream)
     L.nndex_workspace_free(ws)
     L.nndex_gpu_index_free(h)
@@ -186,7 +215,7 @@ def ivf_cpu_baseline(m: np.ndarray, queries: np.ndarray, k: int) -> dict:
 
     L = _lib.lib()
     rows, dims = m.shape
-    threads = min(16, os.cpu_count() or 1)
+    threads, _, _ = host_threads()
     t0 = time.perf_counter()
     ref = oracle.Ivf(m, threads=threads)
     cpu_build = time.perf_counter() - t0
@@ -327,8 +356,6 @@ def serve_bench(args) -> dict:
 
 def precision_bench(args, k: int = 100) -> dict:
     """The precision form (see the module docstring)."""
-    import oracle
-
     L = _lib.lib()
     rows, dims, nq = args.prec_rows, args.prec_dims, args.prec_queries
     t0 = time.time()
@@ -363,13 +390,19 @@ def precision_bench(args, k: int = 100) -> dict:
            "exact": {"qps": round(nq / dt, 1), "ms_per_batch": round(dt * 1e3, 3)}}
     log(f"precision exact: {nq / dt:.0f} QPS ({dt * 1e3:.1f} ms)")
     # FP64 reference on a query sample: largest error of the returned scores
-    sample = list(range(0, nq, max(1, nq // 16)))
-    fp64 = {i: oracle.fp64_scores(base, queries[i], True, threads=min(16, os.cpu_count() or 1))
-            for i in sample}
-    err_exact = max(float(np.max(np.abs(exact_s[i].astype(np.float64) -
-                                         fp64[i][exact_i[i].astype(np.int64)])))
-                    for i in sample)
-    res["exact"]["max_abs_err_vs_fp64"] = err_exact
+    sample = list(range(0, nq, max(1, nq // 64)))
+    res["exact"]["max_abs_err_vs_fp64"] = fp64_err(base, queries, exact_i, exact_s, sample)
+    # FP32 scoring: the fused scan, four queries per launch (the FP32 MFMA
+    # form holds rows of at most 512 padded dims)
+    def fp32_scan():
+        for q0 in range(0, nq, 4):
+            check(L.nndex_gpu_index_search_device(h, ws, dq.at(q0 * dims * 4), 4, k,
+                                                  di.at(q0 * k * 8), ds_.at(q0 * k * 4), stream))
+    dt32 = timed(fp32_scan, reps=1)
+    same = np.array_equal(di.get((nq, k), np.uint64), exact_i)
+    res["fp32_scan"] = {"qps": round(nq / dt32, 1), "ms_per_batch": round(dt32 * 1e3, 3),
+                        "identical_to_exact": bool(same)}
+    log(f"precision fp32 fused scan: {nq / dt32:.0f} QPS ({dt32 * 1e3:.1f} ms), same: {same}")
     for prec, name in ((1, "bf16"), (2, "fp8")):
         dt = timed(lambda: check(L.nndex_gpu_index_search_batch_device(
             h, ws, dq.ptr, nq, k, prec, 4 * k, di.ptr, ds_.ptr, fl.ptr, stream)))
@@ -378,13 +411,13 @@ def precision_bench(args, k: int = 100) -> dict:
         flags = fl.get((nq,), np.uint8)
         recall = float(np.mean([len(set(gi[i].tolist()) & set(exact_i[i].tolist())) / k
                                 for i in range(nq)]))
-        err = max(float(np.max(np.abs(gs[i].astype(np.float64) -
-                                      fp64[i][gi[i].astype(np.int64)]))) for i in sample)
+        err = fp64_err(base, queries, gi, gs, sample)
         res[name] = {"qps": round(nq / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
                      f"recall_at_{k}": round(recall, 5), "overfetch": 4 * k,
                      "certified": round(1.0 - float(flags.mean()), 4),
                      "max_abs_err_vs_fp64": err,
-                     "speedup_vs_exact": round(res["exact"]["ms_per_batch"] / (dt * 1e3), 3)}
+                     "speedup_vs_exact": round(res["exact"]["ms_per_batch"] / (dt * 1e3), 3),
+                     "speedup_vs_fp32_scan": round(dt32 / dt, 3)}
         log(f"precision {name}: {nq / dt:.0f} QPS ({dt * 1e3:.1f} ms), recall@{k} {recall:.5f}, "
             f"certified {1 - flags.mean():.3f}, max |err| vs fp64 {err:.2e}")
     L.nndex_stream_destroy(stream)
@@ -628,21 +661,37 @@ def main():
             flags = fl.get((B,), np.uint8)
             recall = float(np.mean([len(set(gi_b[i].tolist()) & set(exact_i[i].tolist())) / k
                                     for i in range(B)]))
+            gs_b = dsim.get((B, k), np.float32)
+            cert = np.flatnonzero(flags == 0)
+            rc = float(np.mean([len(set(gi_b[i].tolist()) & set(exact_i[i].tolist())) / k
+                                for i in cert])) if len(cert) else 0.0
             batch[name] = {"qps": round(B / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
                            "certified": round(1.0 - float(flags.mean()), 4),
-                           f"recall_at_{k}": round(recall, 5)}
+                           f"recall_at_{k}": round(recall, 5),
+                           f"recall_at_{k}_certified": round(rc, 5),
+                           "max_abs_err_vs_fp64": fp64_err(base, bank, gi_b, gs_b,
+                                                           range(0, B, 64))}
             log(f"{name}: {B / dt:.0f} QPS ({dt * 1e3:.2f} ms), certified "
                 f"{1 - flags.mean():.4f}, recall {recall:.5f}")
+        exact_s = es.get((B, k), np.float32)
+        batch["exact_b4096"]["recall_at_10"] = 1.0  # the exact path by construction
+        batch["exact_b4096"]["max_abs_err_vs_fp64"] = fp64_err(base, bank, exact_i, exact_s,
+                                                               range(0, B, 64))
         if "fp32_b4096" in batch:
+            # BF16 scoring + FP32 rerank against FP32 scoring: the asynchronous
+            # form (uncertified queries flagged) and the exact path (the same
+            # scoring, uncertified queries re-run: recall 1.0 by construction)
             batch["bf16_rerank_speedup_vs_fp32"] = round(
                 batch["bf16_rerank_b4096"]["qps"] / batch["fp32_b4096"]["qps"], 3)
+            batch["exact_bf16_speedup_vs_fp32"] = round(
+                batch["exact_b4096"]["qps"] / batch["fp32_b4096"]["qps"], 3)
 
     # ---- CPU baseline: the oracle's restatement of the reference path ----
     cpu = None
     if not args.no_cpu:
         import oracle
 
-        threads = min(16, os.cpu_count() or 1)
+        threads, nproc, why = host_threads()
         oi, os_, strategy = oracle.cpu_search(base, bank[0], k, threads=threads)
         match = oi.tolist() == gi.tolist() and os_.tolist() == gs.tolist()
         t0 = time.perf_counter()
@@ -653,9 +702,11 @@ def main():
             done += 1
         dt = time.perf_counter() - t0
         cpu = {"value": round(done / dt, 2), "unit": "QPS", "cores": threads, "kind": "port",
+               "nproc": nproc,
                "sample": f"{done} single queries (k={k}, a bank of {nbank} cycled) over the "
                          f"full {rows}x{dims} base, "
-                         f"strategy {strategy}, {dt:.1f} s; results identical to GPU: {match}"}
+                         f"strategy {strategy}, {dt:.1f} s, {threads} threads ({why}; nproc "
+                         f"{nproc}); results identical to GPU: {match}"}
         log(f"cpu baseline: {cpu}")
 
     traffic = None

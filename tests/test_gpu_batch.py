@@ -91,9 +91,11 @@ def test_f32_batch_device_certifies(orc):
 
 @pytest.mark.parametrize("dims,k", [(384, 10), (128, 100), (1024, 10)])
 def test_bf16_rerank_recall_and_certificate(orc, dims, k):
-    """BF16 scoring with 4k over-fetch and exact FP32 rerank: certified queries
-    are identical to exact; recall@k over all queries is reported and must be
-    1.0 for the certified ones."""
+    """BF16 scoring with 4k over-fetch and exact FP32 rerank: a certified
+    query's result is proven identical to the exact FP32 search, so recall@k
+    is 1.0 and the scores are the reference's bits for every certified query
+    (checked against the HIP exact path for all of them and against the
+    oracle's restatement of the reference GPU path for a sample)."""
     _case("bf16", dims=dims, k=k)
     rows = 100_000 if dims < 1024 else 40_000
     base, queries = synth.mixture_workload(rows=rows, dims=dims, n_queries=256)
@@ -103,11 +105,18 @@ def test_bf16_rerank_recall_and_certificate(orc, dims, k):
     ei, es = exact_device(b, ds, queries, k)
     cert = fl == 0
     print(f"certified {cert.mean():.3f}", flush=True)
-    assert cert.mean() >= 0.5
+    assert cert.mean() >= 0.9
     recall = np.mean([len(set(gi[i].tolist()) & set(ei[i].tolist())) / k for i in range(256)])
-    print(f"recall@{k} {recall:.4f}", flush=True)
-    for i in np.nonzero(cert)[0]:
+    print(f"recall@{k} over all queries {recall:.5f}", flush=True)
+    certified = np.nonzero(cert)[0]
+    rc = np.mean([len(set(gi[i].tolist()) & set(ei[i].tolist())) / k for i in certified])
+    assert rc == 1.0  # recall@k of the certified queries
+    for i in certified:
         assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
+    for i in certified[::16]:
+        oidx, osim = orc.gpu_exact_search(base, queries[i], k)
+        assert gi[i].tolist() == oidx.tolist()
+        assert np.array_equal(gs[i].view(np.uint32), osim.view(np.uint32))
 
 
 @pytest.mark.parametrize("dims,k,anisotropic", [(384, 10, False), (128, 100, False),
