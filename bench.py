@@ -426,6 +426,53 @@ def precision_bench(args, k: int = 100) -> dict:
     return res
 
 
+def profile_batch(args) -> None:
+    """Only the batched search at one precision, on the default tile shapes:
+    two untimed batches, then --profile-reps timed ones (HIP events)."""
+    L = _lib.lib()
+    prec = {"f32": 0, "bf16": 1, "fp8": 2}[args.profile_batch]
+    if args.profile_workload == "1m_x_384":
+        base, queries = synth.mixture_workload(n_queries=4096)
+        k, over = 10, 0
+    else:
+        base = synth.anisotropic(5_000_000, 768, seed=42)
+        queries = synth.anisotropic(1024, 768, seed=43)
+        k, over = 100, 400
+    rows, dims = base.shape
+    nq = queries.shape[0]
+    h = ctypes.c_void_p()
+    check(L.nndex_gpu_index_new(base.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), rows, dims,
+                                0, 0, -1, ctypes.byref(h)))
+    del base
+    ws = ctypes.c_void_p()
+    check(L.nndex_workspace_new(h, nq, 448, ctypes.byref(ws)))
+    stream = ctypes.c_void_p()
+    check(L.nndex_stream_create(ctypes.byref(stream)))
+    dq = Dev(queries.nbytes).put(queries)
+    di, ds_, fl = Dev(nq * k * 8), Dev(nq * k * 4), Dev(nq)
+
+    def run():
+        check(L.nndex_gpu_index_search_batch_device(h, ws, dq.ptr, nq, k, prec, over, di.ptr,
+                                                    ds_.ptr, fl.ptr, stream))
+
+    for _ in range(2):
+        run()
+    check(L.nndex_stream_synchronize(stream))
+    e0, e1 = event(), event()
+    check(L.nndex_event_record(e0, stream))
+    for _ in range(args.profile_reps):
+        run()
+    check(L.nndex_event_record(e1, stream))
+    check(L.nndex_stream_synchronize(stream))
+    ms = elapsed(e0, e1) / args.profile_reps
+    flops = 2.0 * nq * rows * dims
+    log(f"profile {args.profile_workload} {args.profile_batch}: {ms:.3f} ms per batch of {nq} "
+        f"({nq / ms * 1e3:.0f} QPS), {flops / ms / 1e9:.0f} TFLOP/s over the whole batch")
+    L.nndex_stream_destroy(stream)
+    L.nndex_workspace_free(ws)
+    L.nndex_gpu_index_free(h)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,6 +514,12 @@ def main():
                     help="request arrivals per second (Poisson)")
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--profile-batch", choices=["f32", "bf16", "fp8"], default=None,
+                    help="only run the batched search at this scoring precision (for profilers)")
+    ap.add_argument("--profile-workload", choices=["1m_x_384", "5m_x_768"], default="1m_x_384",
+                    help="--profile-batch workload: 1M x 384 mixture, 4096 queries, k=10; or "
+                         "5M x 768 anisotropic, 1024 queries, k=100, over-fetch 400")
+    ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: copy calibration + a few searches only")
     args = ap.parse_args()
@@ -476,6 +529,9 @@ def main():
         sys.exit(2)
 
     L = _lib.lib()
+    if args.profile_batch:
+        profile_batch(args)
+        return
     if args.ivf_only:
         print(json.dumps({"ivf": ivf_bench(args)}), flush=True)
         return
