@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // Entries of a wave's LDS candidate queue (12 bytes each).
 constexpr uint32_t kCandQueue = 128;
 
-template <typename E, int NW, int RW, int MAXS, int NT, bool AM>
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
 __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     static_assert(!(AM && E::SCALED), "the row argmax takes unscaled scores");
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
@@ -378,10 +378,18 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     constexpr uint32_t buf_units = q_units + TU;
     const uint32_t block_rows = NW * RW;
     const uint32_t nblk = (a.rows_padded + block_rows - 1) / block_rows;
-    const uint32_t nchunks = (a.nq + QC - 1) / QC;
+    // SPLIT: blockIdx.y takes its own share of the query chunks (a scan
+    // over few rows splits the queries to fill the chip)
+    const uint32_t nch_all = (a.nq + QC - 1) / QC;
+    uint32_t ch0 = 0, ch1 = nch_all;
+    if constexpr (SPLIT) {
+        const uint32_t nch_per = (nch_all + gridDim.y - 1) / gridDim.y;
+        ch0 = blockIdx.y * nch_per;
+        ch1 = ch0 + nch_per < nch_all ? ch0 + nch_per : nch_all;
+    }
     constexpr uint32_t RU = 4 * MAXS;    // staged units per query (>= dch)
     constexpr uint32_t chunk_units = QC * RU;
-    if (a.nq == 0 || dch == 0) return;
+    if (a.nq == 0 || dch == 0 || ch0 >= ch1) return;
     auto thl = [&](uint32_t buf) {
         return reinterpret_cast<unsigned long long *>(qlds + buf * buf_units + q_units);
     };
@@ -556,15 +564,16 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 }
             }
         __syncthreads();  // previous block tile done with both buffers
-        fetch(0);
-        commit(0, 0);
+        fetch(ch0);
+        commit(0, ch0);
         __syncthreads();
-        for (uint32_t ch = 0; ch < nchunks; ++ch) {
-            const bool more = ch + 1 < nchunks;
+        for (uint32_t ch = ch0; ch < ch1; ++ch) {
+            const bool more = ch + 1 < ch1;
+            const uint32_t cur = (ch - ch0) & 1;
             fetch(more ? ch + 1 : ch);
-            const unsigned long long *thc = thl(ch & 1);
-            const uint4 *src = qlds + (ch & 1) * buf_units;
-            if (late && ch > 0) epi0(accp, thkp, ch - 1, bt, rs);
+            const unsigned long long *thc = thl(cur);
+            const uint4 *src = qlds + cur * buf_units;
+            if (late && ch > ch0) epi0(accp, thkp, ch - 1, bt, rs);
             v4f acc[MT][NT][E::NACC];
 #pragma unroll
             for (int m = 0; m < MT; ++m)
@@ -667,11 +676,11 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 }
             }
-            if (more) commit((ch + 1) & 1, ch + 1);
+            if (more) commit(cur ^ 1, ch + 1);
             __syncthreads();
         }
         if constexpr (!AM) {
-            if (late) epi0(accp, thkp, nchunks - 1, bt, rs);
+            if (late) epi0(accp, thkp, ch1 - 1, bt, rs);
             if (cq_n) flush();
         }
         if constexpr (AM) {
@@ -767,6 +776,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     __shared__ uint64_t slots[4][CW];
     __shared__ uint32_t cnt[4];
     __shared__ uint64_t exact[CW];
+    // per wave: 64 rows x (8 + 1) float4 (the large over-fetch form only)
+    __shared__ float4 stage[CW >= 512 ? 4 : 1][CW >= 512 ? 64 * 9 : 1];
     const uint32_t q = blockIdx.x, lane = __lane_id(), wid = threadIdx.x >> 6;
     const uint32_t total = a.cand_count[q];
     const bool overflow = total > a.capq;
@@ -802,24 +813,76 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     __syncthreads();
     const uint32_t m = cnt[0];                       // approx list length (<= k'+1)
     const uint32_t nres = m < a.kp ? m : a.kp;       // rescored
-    // exact rescoring, one thread per candidate, in the reference dot order
-    for (uint32_t i = threadIdx.x; i < CW; i += 256) {
-        uint64_t key = 0;
-        if (i < nres) {
-            const uint32_t row = key_row(slots[0][i]);
-            const float4 *xb = a.x + (size_t)row * a.d4;
-            const float4 *qb = a.q + (size_t)q * a.d4;
+    // Exact rescoring in the reference dot order, one lane per candidate.
+    if constexpr (CW >= 512) {
+        // Large over-fetch: each wave copies its 64 candidates' rows
+        // through LDS in segments of kSeg float4 (kSeg lanes read one row's
+        // 128-byte run, rather than 64 lanes each reading its own row),
+        // then every lane runs its row's four FMA chains over the
+        // segment.  The next segment's loads are in flight while the
+        // current one is summed.
+        constexpr uint32_t kSeg = 8, kPad = kSeg + 1;  // padded rows: no bank conflicts
+        float4 *stg = stage[CW >= 512 ? wid : 0];
+        const float4 *qrow = a.q + (size_t)q * a.d4;
+        for (uint32_t i0 = 0; i0 < CW; i0 += 256) {
+            const uint32_t wbase = i0 + wid * 64;
+            const uint32_t i = wbase + lane;
             float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
-            for (uint32_t cc = 0; cc < a.d4; ++cc) {
-                float4 xv = xb[cc], qv = qb[cc];
-                l0 = __builtin_fmaf(xv.x, qv.x, l0);
-                l1 = __builtin_fmaf(xv.y, qv.y, l1);
-                l2 = __builtin_fmaf(xv.z, qv.z, l2);
-                l3 = __builtin_fmaf(xv.w, qv.w, l3);
+            if (wbase < nres) {
+                // the rows this lane copies: candidate 8 j + lane / 8 of the wave
+                uint32_t crow[8];
+    #pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t ci = wbase + 8 * j + lane / kSeg;
+                    crow[j] = ci < nres ? key_row(slots[0][ci]) : 0xFFFFFFFFu;
+                }
+                float4 v[8];
+                auto load = [&](uint32_t c0) {
+                    const uint32_t cc = c0 + lane % kSeg;
+    #pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        v[j] = crow[j] != 0xFFFFFFFFu && cc < a.d4 ? a.x[(size_t)crow[j] * a.d4 + cc]
+                                                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+                };
+                load(0);
+                for (uint32_t c0 = 0; c0 < a.d4; c0 += kSeg) {
+                    wave_lds_sync();  // the previous segment is summed
+    #pragma unroll
+                    for (int j = 0; j < 8; ++j) stg[(8 * j + lane / kSeg) * kPad + lane % kSeg] = v[j];
+                    wave_lds_sync();
+                    if (c0 + kSeg < a.d4) load(c0 + kSeg);
+                    const uint32_t nc = a.d4 - c0 < kSeg ? a.d4 - c0 : kSeg;
+                    for (uint32_t cc = 0; cc < nc; ++cc) {
+                        const float4 xv = stg[lane * kPad + cc], qv = qrow[c0 + cc];
+                        l0 = __builtin_fmaf(xv.x, qv.x, l0);
+                        l1 = __builtin_fmaf(xv.y, qv.y, l1);
+                        l2 = __builtin_fmaf(xv.z, qv.z, l2);
+                        l3 = __builtin_fmaf(xv.w, qv.w, l3);
+                    }
+                }
             }
-            key = make_key((l0 + l1) + (l2 + l3), row);
+            if (i < CW) exact[i] = i < nres ? make_key((l0 + l1) + (l2 + l3), key_row(slots[0][i])) : 0;
         }
-        exact[i] = key;
+    } else {
+        // a few rows per query: one thread streams each candidate row
+        for (uint32_t i = threadIdx.x; i < CW; i += 256) {
+            uint64_t key = 0;
+            if (i < nres) {
+                const uint32_t row = key_row(slots[0][i]);
+                const float4 *xb = a.x + (size_t)row * a.d4;
+                const float4 *qb = a.q + (size_t)q * a.d4;
+                float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+                for (uint32_t cc = 0; cc < a.d4; ++cc) {
+                    float4 xv = xb[cc], qv = qb[cc];
+                    l0 = __builtin_fmaf(xv.x, qv.x, l0);
+                    l1 = __builtin_fmaf(xv.y, qv.y, l1);
+                    l2 = __builtin_fmaf(xv.z, qv.z, l2);
+                    l3 = __builtin_fmaf(xv.w, qv.w, l3);
+                }
+                key = make_key((l0 + l1) + (l2 + l3), row);
+            }
+            exact[i] = key;
+        }
     }
     __syncthreads();
     if (wid == 0) {
@@ -1062,12 +1125,12 @@ void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, f
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
-static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
+static void launch_mfma_split(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     // two query/threshold buffers, then (mode 0) the waves' candidate queues
     const size_t lds = (2ull * (MAXS * 4 * 16 * NT + 8 * NT) +
                         (AM ? 0 : NW * kCandQueue * 12 / 16)) * sizeof(uint4);
-    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM>;
+    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM, SPLIT>;
     // once per instantiation (also keeps the call out of graph captures);
     // serving executors launch from several threads, so the first call sets
     // it and every other waits until it is set
@@ -1078,8 +1141,16 @@ static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     });
     NNDEX_HIP_CHECK(attr_err);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(NW * 64), lds, s, a);
+    hipLaunchKernelGGL(fn, dim3(grid, SPLIT ? a.qsplit : 1), dim3(NW * 64), lds, s, a);
     NNDEX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
+static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
+    if constexpr (!AM) {
+        if (a.qsplit > 1) return launch_mfma_split<E, NW, RW, MAXS, NT, AM, true>(a, grid, s);
+    }
+    launch_mfma_split<E, NW, RW, MAXS, NT, AM, false>(a, grid, s);
 }
 
 // Tile shapes per step count (a row of `dch` 16-byte units runs the smallest

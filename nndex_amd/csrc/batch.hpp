@@ -34,6 +34,9 @@ struct MfmaScanArgs {
     const float *tol;          // device scalar
     uint32_t *recheck;         // [rows]
     uint32_t *recheck_count;
+    // query splits (grid y): each block row scans its share of the query
+    // chunks (0 or 1: one share)
+    uint32_t qsplit;
 };
 
 struct RerankArgs {

@@ -426,9 +426,25 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.ldo = static_cast<uint32_t>(sample_ld(smp.rows));
         const int variant = mfma_variant_[elem].load(std::memory_order_relaxed);
         const uint32_t br = mfma_block_rows(elem, dch, false, variant);
+        // one 8-wave block per CU (register-bound); a scan with fewer row
+        // tiles than CUs also splits the queries over blocks (grid y),
+        // choosing the split that finishes in the fewest block waves per
+        // unit of work
         auto grid_for = [&](uint32_t rp) {
-            uint32_t blocks = (rp + br - 1) / br;
-            uint32_t cap = ncu_;  // one 8-wave block per CU (register-bound)
+            const uint32_t blocks = (rp + br - 1) / br;
+            const uint32_t cap = ncu_;
+            a.qsplit = 1;
+            if (blocks < cap) {
+                const uint32_t max_split = static_cast<uint32_t>((n + 15) / 16);
+                double best = 1.0;
+                for (uint32_t q = 2; q <= 8 && q <= max_split; ++q) {
+                    const double t = (double)((blocks * q + cap - 1) / cap) / q;
+                    if (t < best - 1e-9) {
+                        best = t;
+                        a.qsplit = q;
+                    }
+                }
+            }
             return blocks < cap ? blocks : cap;
         };
         launch_mfma_scan(a, elem, grid_for(a.rows_padded), s, variant);
