@@ -128,6 +128,16 @@ def ivf_bench(args, k: int = 10) -> dict:
     """The IVF form on a resident index (see the module docstring)."""
     L = _lib.lib()
     rows, dims, nlist, nq = args.ivf_rows, 128, args.ivf_nlist, args.ivf_queries
+    if args.profile:
+        # a plain copy of known size, whose FETCH_SIZE / WRITE_SIZE counts
+        # scale the list scan's counters to bytes
+        copy_bytes = 1 << 30
+        csrc, cdst, cs = Dev(copy_bytes), Dev(copy_bytes), ctypes.c_void_p()
+        check(L.nndex_stream_create(ctypes.byref(cs)))
+        check(L.nndex_device_copy(cdst.ptr, csrc.ptr, copy_bytes, cs))
+        check(L.nndex_stream_synchronize(cs))
+        L.nndex_stream_destroy(cs)
+        log(f"ivf profile: copy kernel over {copy_bytes} bytes done")
     t0 = time.time()
     centres = synth.mixture_centres(dims, 4096, seed=7)
     base = synth.gaussian_mixture(rows, dims, centres, 0.3, 42)
@@ -521,7 +531,8 @@ def main():
                          "5M x 768 anisotropic, 1024 queries, k=100, over-fetch 400")
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--profile", action="store_true",
-                    help="short run for profilers: copy calibration + a few searches only")
+                    help="short run for profilers: copy calibration + a few searches only "
+                         "(with --ivf-only: a copy calibration before the IVF form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "

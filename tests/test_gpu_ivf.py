@@ -73,6 +73,7 @@ def test_ivf_build_equals_oracle(orc, rows, dims, nlist, iters):
                "found yet (the same library passed it before), so it stays off the GPU")),
     (4096, 128, 1, 1),
     (6000, 32, 100, 0),
+    (4000, 32, 10, 60),   # 12 queries x 60 probes over 63 lists: full 8-query wavefronts
     (3000, 1536, 10, 0),
     (2600, 3072, 7, 4),
 ])
@@ -286,17 +287,20 @@ def test_ivf_save_and_load_skip_the_build(orc, tmp_path):
     assert np.array_equal(b_one[1].view(np.uint32), rsim.view(np.uint32))
 
 
-@pytest.mark.parametrize("k,nprobe", [(481, 20), (1000, 20), (3000, 24), (1000, 2)])
-def test_ivf_large_k_equals_oracle(orc, k, nprobe):
+@pytest.mark.parametrize("k,nprobe,rows,nq", [(481, 20, 30_000, 7), (1000, 20, 30_000, 7),
+                                              (3000, 24, 30_000, 7), (1000, 2, 30_000, 7),
+                                              (600, 70, 6_000, 14)])
+def test_ivf_large_k_equals_oracle(orc, k, nprobe, rows, nq):
     """k beyond the list scan's LDS lists (2k - 1 > 960): every probed row's
     key, a radix select of the top 2k - 1 per query and the heap's tie rule
     over arrival order.  nprobe 2 probes fewer rows than k (results = the
     probed rows).  Single and batch searches equal the oracle bit for bit,
-    padding past the probed rows included."""
-    rows, dims = 30_000, 32
-    _case("ivf_large_k", rows=rows, dims=dims, k=k, nprobe=nprobe)
+    padding past the probed rows included.  The 6,000-row case fills the
+    list scan's 8-query wavefronts."""
+    dims = 32
+    _case("ivf_large_k", rows=rows, dims=dims, k=k, nprobe=nprobe, nq=nq)
     m, rng = _storage(orc, rows, dims, 5 + k, mixture=True)
-    qs = orc.normalize_rows(rng.normal(size=(7, dims)).astype(np.float32), 7, dims)
+    qs = orc.normalize_rows(rng.normal(size=(nq, dims)).astype(np.float32), nq, dims)
     b = BackendIndex(m, approx=True)
     b.set_ivf(nprobe=nprobe)
     ref = orc.Ivf(m)

@@ -122,6 +122,8 @@ def _ivf(orc, rows, dims, seed):
     (30000, 32, 481, 20, 7),   # the large-k form whose key slot faulted
     (30000, 32, 3000, 24, 1),
     (6000, 32, 100, 52, 12),
+    (4000, 32, 10, 40, 24),    # full 8-query wavefronts
+    (9000, 32, 700, 60, 40),   # full 8-query wavefronts in the large-k form
 ])
 def test_ivf_scan_indices_stay_in_bounds(orc, rows, dims, k, nprobe, nq):
     m, rng, ref = _ivf(orc, rows, dims, 3 * rows + dims)
