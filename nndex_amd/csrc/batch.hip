@@ -57,7 +57,8 @@ struct F32E {
     // the skewed epilogue only pays where MFMA steps are short (16-bit and
     // 8-bit forms); four f32 MFMAs per step already hide the epilogue
     static constexpr bool SKEW = false;
-    static __device__ inline void mma2(uint4, uint4, uint4, uint4, v4f (&)[NACC]) {}
+    static __device__ inline void mma2(uint4, uint4, uint4, uint4, uint32_t, uint32_t,
+                                       v4f (&)[NACC]) {}
 };
 struct BF16E {
     // one 16x16x32 bf16 MFMA over the unit's eight bf16 values
@@ -68,23 +69,27 @@ struct BF16E {
     }
     static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
     static constexpr bool SCALED = false, PAIR = false, SKEW = true;
-    static __device__ inline void mma2(uint4, uint4, uint4, uint4, v4f (&)[NACC]) {}
+    static __device__ inline void mma2(uint4, uint4, uint4, uint4, uint32_t, uint32_t,
+                                       v4f (&)[NACC]) {}
 };
 struct FP8E {
-    // one 16x16x128 f8f6f4 MFMA (both operands OCP E4M3, unit block scales)
-    // per two 16-byte steps: lane group g supplies the 32 codes of its units
-    // 4s+g and 4(s+1)+g, rows and queries alike, so the K order is the same
-    // on both sides; the score is rescaled by the row and query scales in the
-    // epilogue
+    // one 16x16x128 f8f6f4 MFMA (both operands OCP E4M3) per two 16-byte
+    // steps: lane group g supplies the 32 codes of its units 4s+g and
+    // 4(s+1)+g, rows and queries alike, so the K order is the same on both
+    // sides.  Row and query scales are powers of two, applied by the MFMA
+    // itself as E8M0 block scales (sa: the lane's row, sb: the lane's query;
+    // the same for every K block), so the scores come out scaled.
     static constexpr int NACC = 1;
     static constexpr bool SCALED = true, PAIR = true, SKEW = true;
     static __device__ inline void mma(uint4, uint4, v4f (&)[NACC]) {}
-    static __device__ inline void mma2(uint4 a0, uint4 a1, uint4 b0, uint4 b1, v4f (&c)[NACC]) {
+    static __device__ inline void mma2(uint4 a0, uint4 a1, uint4 b0, uint4 b1, uint32_t sa,
+                                       uint32_t sb, v4f (&c)[NACC]) {
         const v8i A = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
                        (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
         const v8i B = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
                        (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-        c[0] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(A, B, c[0], 0, 0, 0, 127, 0, 127);
+        c[0] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(A, B, c[0], 0, 0, 0, (int)sa, 0,
+                                                                 (int)sb);
     }
     static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
 };
@@ -118,6 +123,117 @@ __device__ inline float fp8_e4m3_value(uint32_t c) {
     return (c & 0x80u) ? -v : v;
 }
 
+// Scale of an FP8 row or query: the least power of two p >= max|v| / 448, so
+// every code lies within the E4M3 range and p is an E8M0 block scale (E4M3
+// is floating point: a power-of-two scale costs no precision); kept at or
+// above 2^-126 (a normal float), 0 for an all-zero vector.
+__host__ __device__ inline float fp8_pow2_scale(float mx) {
+    if (!(mx > 0.f)) return 0.f;
+    const float r = mx / 448.0f;
+    const uint32_t b = __builtin_bit_cast(uint32_t, r);
+    uint32_t e = (b >> 23) & 0xFFu;                  // r = 1.m x 2^(e - 127)
+    if (e == 0) return 0x1p-126f;                    // subnormal r: p = 2^-126
+    if ((b & 0x7FFFFFu) != 0) ++e;                   // not a power of two: round up
+    if (e > 254) e = 254;
+    return __builtin_bit_cast(float, e << 23);
+}
+// Bound on |result - exact| / sum |a_k b_k| of one FP8 MFMA over its 128
+// products (v_mfma_scale_f32_16x16x128_f8f6f4).  Its accumulation is not
+// FP32-exact: on gfx950 random and wide-range E4M3 inputs measured up to
+// 1.4e-4 (2^-12.8) of the absolute sum, 18x the FP32 bound 128 u; the bound
+// used is 2^-8, 28x the largest measured.  The FP32 forms (16x16x4f32) and
+// BF16 (16x16x32) measured within their FP32 bounds.
+constexpr double kFp8MfmaRelErr = 0x1p-8;
+// E8M0 byte of a power-of-two scale (127 for a zero scale: its codes are 0).
+__device__ inline uint32_t e8m0_of(float p) {
+    const uint32_t e = (__builtin_bit_cast(uint32_t, p) >> 23) & 0xFFu;
+    return e ? e : 127u;
+}
+
+// ---------------------------------------------------------------------------
+// Global -> LDS copies without registers (global_load_lds_dwordx4): lane L of
+// the wave copies its 16 bytes at `g` to LDS byte address m + 16 L (m is
+// wave-uniform, taken through M0).  Written in asm so the compiler does not
+// order every later LDS read after the copy (it cannot tell the buffers
+// apart): completion is tracked by the vector memory counter, and the scan
+// waits for it explicitly (wait_vm) before the barrier that publishes a
+// buffer.  M0 is restored afterwards; one wait state separates its write
+// from the copy.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t lds_off(const void *p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+__device__ inline void dma16(const uint4 *g, uint32_t m) {
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %1\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %2, off\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "s"(__builtin_amdgcn_readfirstlane(m)), "v"(g)
+        : "memory");
+}
+// s_waitcnt immediate of vmcnt(0) alone (expcnt 7, lgkmcnt 15: no wait on
+// those), for the compiler's own builtin, whose waits it takes into account.
+constexpr int kWaitVm0 = 0x0F70;
+// Wait until at most N vector memory operations of this wave are in flight
+// (they complete in issue order).
+template <int N>
+__device__ inline void wait_vm() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+// The same for a wave-uniform count; above 16 it waits for 16 (stricter).
+__device__ inline void wait_vm_dyn(uint32_t n) {
+    switch (n) {
+    case 0: wait_vm<0>(); break;
+    case 1: wait_vm<1>(); break;
+    case 2: wait_vm<2>(); break;
+    case 3: wait_vm<3>(); break;
+    case 4: wait_vm<4>(); break;
+    case 5: wait_vm<5>(); break;
+    case 6: wait_vm<6>(); break;
+    case 7: wait_vm<7>(); break;
+    case 8: wait_vm<8>(); break;
+    case 9: wait_vm<9>(); break;
+    case 10: wait_vm<10>(); break;
+    case 11: wait_vm<11>(); break;
+    case 12: wait_vm<12>(); break;
+    case 13: wait_vm<13>(); break;
+    case 14: wait_vm<14>(); break;
+    case 15: wait_vm<15>(); break;
+    default: wait_vm<16>(); break;
+    }
+}
+
+// Chunk image of a query batch for the MFMA scan's LDS copies: chunk ch
+// (queries ch*QC .. ch*QC + QC - 1) is QC*RU 16-byte units laid out exactly
+// as the scan's LDS buffer: unit row c (RU of them, units c >= dch zero),
+// query n at slot (n & ~15) | ((n ^ c) & 15); queries past nq are zero.  The
+// thresholds and scales of the padding queries (up to nch*QC) are set so they
+// pass nothing: threshold floors +inf, scales 0.
+__global__ __launch_bounds__(256) void chunk_image_kernel(const uint4 *__restrict__ q,
+                                                          uint32_t q_stride, uint32_t nq,
+                                                          uint32_t dch, uint32_t qc, uint32_t ru,
+                                                          uint32_t nch, uint4 *__restrict__ img,
+                                                          float *thresh_pad, float *scale_pad) {
+    const size_t per = (size_t)qc * ru, total = per * nch;
+    for (size_t u = (size_t)blockIdx.x * 256 + threadIdx.x; u < total;
+         u += (size_t)gridDim.x * 256) {
+        const uint32_t ch = (uint32_t)(u / per), r = (uint32_t)(u % per);
+        const uint32_t c = r / qc, slot = r % qc;
+        const uint32_t n = (slot & ~15u) | ((slot ^ c) & 15u);
+        const uint32_t qi = ch * qc + n;
+        img[u] = (qi < nq && c < dch) ? q[(size_t)qi * q_stride + c] : make_uint4(0, 0, 0, 0);
+    }
+    if (blockIdx.x == 0)
+        for (uint32_t i = nq + threadIdx.x; i < nch * qc; i += 256) {
+            if (thresh_pad) thresh_pad[i] = __builtin_inff();
+            if (scale_pad) scale_pad[i] = 0.f;
+        }
+}
+
 // Tiled FP32 -> tiled FP8 with a per-row scale; one thread per row (lanes of a
 // wave read and write 64 consecutive rows of a tile: both coalesced).
 __global__ __launch_bounds__(256) void to_fp8_kernel(const float4 *__restrict__ x,
@@ -137,7 +253,7 @@ __global__ __launch_bounds__(256) void to_fp8_kernel(const float4 *__restrict__ 
                                                      __builtin_fmaxf(__builtin_fabsf(v.z),
                                                                     __builtin_fabsf(v.w))));
         }
-        const float sc = mx > 0.f ? mx / 448.0f : 0.f;
+        const float sc = fp8_pow2_scale(mx);
         const uint32_t d16 = (d4 + 3) / 4;
         uint4 *o = y + (size_t)(row / 64) * d16 * 64 + row % 64;
         for (uint32_t u = 0; u < d16; ++u) {
@@ -243,11 +359,11 @@ __global__ __launch_bounds__(256) void max_norm_kernel(const float4 *__restrict_
 
 // ---------------------------------------------------------------------------
 // Query preparation: low-precision copy (row-major, 16-byte units; FP8 with a
-// per-query scale s_q = max|q| / 448) and the per-query error bound delta of
-// |approximate - exact| over any row, with x~, q~ the dequantised copies
-// (x~ = x, q~ = q for FP32):
+// per-query power-of-two scale s_q >= max|q| / 448) and the per-query error
+// bound delta of |approximate - exact| over any row, with x~, q~ the
+// dequantised copies (x~ = x, q~ = q for FP32):
 //   exact side   gamma_n |x| |q|                 (four-lane FMA chain)
-//   approx side  gamma_n |x~| |q~| (+ 3u |x~| |q~| for the FP8 rescaling)
+//   approx side  gamma_n |x~| |q~|  (FP8: kFp8MfmaRelErr + gamma_(n/128+1))
 //   rounding     |x~| |q~ - q| + |x~ - x| |q|
 // with |x| <= xmax (largest row norm), |x~ - x| <= xe (largest row rounding
 // error, measured when the copy was made) and |x~| <= xmax + xe.
@@ -275,7 +391,7 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
                 smax[threadIdx.x] = __builtin_fmaxf(smax[threadIdx.x], smax[threadIdx.x + st]);
             __syncthreads();
         }
-        sq = smax[0] > 0.f ? smax[0] / 448.0f : 0.f;
+        sq = fp8_pow2_scale(smax[0]);
         if (threadIdx.x == 0) q_scale[qi] = sq;
     }
     double nq2 = 0, nl2 = 0, nd2 = 0;
@@ -336,7 +452,11 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
             xe = sqrt(__builtin_bit_cast(double, *xerr_sq));
         }
         const double xl = xm + xe;
-        const double ga = elem == kElemFP8 ? gamma + 3.0 * u : gamma;
+        // the FP8 MFMA does not sum its 128 products within FP32's bound:
+        // its error is bounded by kFp8MfmaRelErr of their absolute sum per
+        // instruction, then FP32 accumulation across the n/128 instructions
+        const double nc = ceil(n / 128.0) + 1.0;
+        const double ga = elem == kElemFP8 ? kFp8MfmaRelErr + nc * u / (1.0 - nc * u) : gamma;
         const double d = gamma * xm * qn + ga * xl * ql + xl * qd + xe * qn;
         delta[qi] = (float)(d * (1.0 + 1e-6)) + 1e-30f;
     }
@@ -346,17 +466,45 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // X-stationary MFMA scan.  NW waves per block, RW rows per wave (multiple of
 // 16), MAXS 16-byte units per lane per row (units per row <= 4 * MAXS; lane
 // group g = lane/16 owns units 4s+g), NT 16-query MFMA tiles per query chunk.
-// The next chunk's queries and thresholds are fetched into registers while
-// the current chunk runs on the MFMA units, then written to the other LDS
-// buffer.  Mode 0 appends candidates, mode 1 writes scores; the AM variant
-// (mode 2) keeps each row's two best queries and writes the row argmax.
+// Mode 0 appends candidates, mode 1 writes scores: their query chunks come
+// from the batch's chunk image (chunk_image_kernel) by LDS copies issued
+// NB - 1 chunks ahead into a ring of NB buffers, so a chunk's copies have
+// NB - 2 chunks of MFMA work to land in and no registers hold them.  The
+// AM variant (mode 2) keeps each row's two best queries and writes the row
+// argmax; it reads row-major queries (centroids) and stages the next chunk
+// through registers into the other of two buffers.
 // ---------------------------------------------------------------------------
 // Entries of a wave's LDS candidate queue (12 bytes each).
 constexpr uint32_t kCandQueue = 128;
 
+// LDS of one query buffer in 16-byte units: [4*steps unit rows][QC slots] of
+// query units, then the chunk's thresholds (candidate mode: QC/4 units of
+// float floors; the row argmax's register staging keeps QC/2 units of
+// 64-bit keys) and QC/4 units of query scales for the scaled (FP8) form.
+template <typename E, int MAXS, int NT, bool AM>
+constexpr uint32_t mfma_buf_units() {
+    return (uint32_t)(MAXS * 4 * 16 * NT + (AM ? 8 * NT : 4 * NT) + (E::SCALED ? 4 * NT : 0));
+}
+// Buffers in the ring: as many as fit 160 KiB beside the candidate queues
+// (at most 4); the row argmax keeps its two.
+template <typename E, int NW, int MAXS, int NT, bool AM>
+constexpr uint32_t mfma_nbuf() {
+    if (AM) return 2;
+    constexpr size_t queue = (size_t)NW * kCandQueue * 12;
+    constexpr size_t b = (size_t)mfma_buf_units<E, MAXS, NT, AM>() * 16, lds = 160 * 1024;
+    return 4 * b + queue <= lds ? 4 : (3 * b + queue <= lds ? 3 : 2);
+}
+template <typename E, int NW, int MAXS, int NT, bool AM>
+constexpr size_t mfma_lds_bytes() {
+    return ((size_t)mfma_nbuf<E, NW, MAXS, NT, AM>() * mfma_buf_units<E, MAXS, NT, AM>() +
+            (AM ? 0 : NW * kCandQueue * 12 / 16)) *
+           sizeof(uint4);
+}
+
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
 __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     static_assert(!(AM && E::SCALED), "the row argmax takes unscaled scores");
+    constexpr bool DMA = !AM;            // LDS copies from the chunk image
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
     constexpr uint32_t QC = 16 * NT;     // queries per chunk
     // LDS slots per unit row: unpadded, so the 16 lanes of a group read one
@@ -366,16 +514,21 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     constexpr uint32_t kStride = QC;
     constexpr uint32_t NTH = NW * 64;
     constexpr int STG = (int)((QC * 4 * MAXS + NTH - 1) / NTH);  // staged units per thread
-    constexpr uint32_t TU = QC / 2;      // threshold units (two 64-bit keys each) per buffer
-    extern __shared__ uint4 qlds[];      // 2 buffers x ([4*steps units][kStride] + TU)
+    // threshold units per buffer: four float floors each (LDS copies), or
+    // two 64-bit keys each (register staging)
+    constexpr uint32_t TU = DMA ? QC / 4 : QC / 2;
+    constexpr uint32_t SU = E::SCALED ? QC / 4 : 0;  // query-scale units per buffer
+    constexpr uint32_t NB = mfma_nbuf<E, NW, MAXS, NT, AM>();
+    extern __shared__ uint4 qlds[];      // NB buffers x ([4*steps units][kStride] + TU + SU)
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
-    const uint32_t wid = tid >> 6;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t g = lane >> 4, l16 = lane & 15;
     const uint32_t dch = a.dch;
     constexpr uint32_t steps = MAXS;  // launcher guarantees 4 * MAXS >= dch
     constexpr uint32_t q_units = steps * 4 * kStride;
-    constexpr uint32_t buf_units = q_units + TU;
+    constexpr uint32_t buf_units = q_units + TU + SU;
+    static_assert(buf_units == mfma_buf_units<E, MAXS, NT, AM>(), "LDS layout");
     const uint32_t block_rows = NW * RW;
     const uint32_t nblk = (a.rows_padded + block_rows - 1) / block_rows;
     // SPLIT: blockIdx.y takes its own share of the query chunks (a scan
@@ -393,12 +546,18 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     auto thl = [&](uint32_t buf) {
         return reinterpret_cast<unsigned long long *>(qlds + buf * buf_units + q_units);
     };
-    // Mode 0 candidate queue of the wave in LDS (after both buffers): keys
+    auto thfl = [&](uint32_t buf) {
+        return reinterpret_cast<const float *>(qlds + buf * buf_units + q_units);
+    };
+    auto qsl = [&](uint32_t buf) {
+        return reinterpret_cast<const float *>(qlds + buf * buf_units + q_units + TU);
+    };
+    // Mode 0 candidate queue of the wave in LDS (after the buffers): keys
     // and query ids of the pairs that passed, flushed to the per-query lists
     // when full and at the end of each row tile, so the epilogue never waits
     // on a returning atomic.
-    uint64_t *cq_key = reinterpret_cast<uint64_t *>(qlds + 2 * buf_units) + wid * kCandQueue;
-    uint32_t *cq_qi = reinterpret_cast<uint32_t *>(qlds + 2 * buf_units + NW * kCandQueue / 2) +
+    uint64_t *cq_key = reinterpret_cast<uint64_t *>(qlds + NB * buf_units) + wid * kCandQueue;
+    uint32_t *cq_qi = reinterpret_cast<uint32_t *>(qlds + NB * buf_units + NW * kCandQueue / 2) +
                       wid * kCandQueue;
     uint32_t cq_n = 0;  // wave-uniform
     auto flush = [&]() {
@@ -413,11 +572,38 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
         cq_n = 0;
     };
 
-    // zero the padding units of both buffers once (units >= dch)
-    for (uint32_t i = tid; i < 2 * buf_units; i += NTH) {
-        const uint32_t r = i % buf_units;
-        if (r < q_units && r / kStride >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
+    // register staging (row argmax): zero the padding units of both buffers
+    // once (units >= dch); the chunk image carries its own zeros
+    if constexpr (!DMA) {
+        for (uint32_t i = tid; i < 2 * buf_units; i += NTH) {
+            const uint32_t r = i % buf_units;
+            if (r < q_units && r / kStride >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
+        }
     }
+    // LDS copies of one chunk from the image: NI copies of 1 KiB (64 units)
+    // for the queries, one of QC float thresholds (mode 0) and one of QC
+    // scales (scaled form), copy i issued by wave i % NW; my_cnt = this
+    // wave's.
+    constexpr uint32_t NI = (uint32_t)(NT * MAXS);
+    const bool stage_th = a.mode == 0;
+    const uint32_t my_cnt = (NI > wid ? (NI - wid + NW - 1) / NW : 0) +
+                            ((stage_th && wid == NI % NW) ? 1u : 0u) +
+                            ((E::SCALED && wid == (NI + 1) % NW) ? 1u : 0u);
+    const uint32_t lds0 = __builtin_amdgcn_readfirstlane(lds_off(qlds));
+    auto dma_chunk = [&](uint32_t ch, uint32_t buf) {
+        const uint32_t dst = lds0 + buf * buf_units * 16;
+        const uint4 *src = a.qimg + (size_t)ch * q_units + lane;
+        for (uint32_t i = wid; i < NI; i += NW) dma16(src + i * 64, dst + i * 64 * 16);
+        // TU >= 4 and SU >= 4 lanes take part: never an empty copy
+        if (stage_th && wid == NI % NW && lane < TU)
+            dma16(reinterpret_cast<const uint4 *>(a.thresh_f + (size_t)ch * QC) + lane,
+                  dst + q_units * 16);
+        if constexpr (E::SCALED) {
+            if (wid == (NI + 1) % NW && lane < SU)
+                dma16(reinterpret_cast<const uint4 *>(a.q_scale + (size_t)ch * QC) + lane,
+                      dst + (q_units + TU) * 16);
+        }
+    };
 
     // Staging of the next chunk (queries and their thresholds).  The loads
     // are unconditional (clamped addresses; out-of-range values are dropped
@@ -467,14 +653,16 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // (finite inputs).  A key at or above the threshold has a score at or
     // above the threshold's floor, so each 16-query tile first tests the
     // largest score of every lane and is skipped when none is that high.
-    auto epi0 = [&](const v4f (&ac)[MT][NT][E::NACC], const unsigned long long (&thk)[NT],
-                    uint32_t ch, uint32_t bt, const float (&rs)[MT][4]) {
+    auto epi0 = [&](const v4f (&ac)[MT][NT][E::NACC], const float (&thv)[NT], uint32_t ch,
+                    uint32_t bt) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const uint32_t qi = ch * QC + t * 16 + l16;
-            float qs = 1.f;
-            if constexpr (E::SCALED) qs = qi < a.nq ? a.q_scale[qi] : 0.f;
-            const float thf = key_floor_sim(thk[t]);
+            // the threshold key's floor: every threshold key holds a score
+            // in its upper half and zero below, so a score passes exactly
+            // when it is at or above the floor (-0 and +0 aside, which only
+            // adds a candidate); +inf for the padding queries
+            const float thf = thv[t];
             float sc[MT][4];
             float mx = -__builtin_inff();
 #pragma unroll
@@ -482,38 +670,42 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     sc[m][v] = E::total(ac[m][t], v);
-                    if constexpr (E::SCALED) sc[m][v] = sc[m][v] * (qs * rs[m][v]);
                     mx = __builtin_fmaxf(mx, sc[m][v]);
                 }
             if (__ballot(!(mx < thf)) == 0) continue;
+            // Rare from here: the lane's (row, query) pairs at or above the
+            // threshold as bits m*4+v, then appended a round at a time (one
+            // pair per lane per round), which keeps this path short in code
+            // and in time.  Rows past the index and queries past the batch
+            // never pass.
+            const uint32_t row0 = bt * block_rows + wid * RW + 4 * g;
+            uint32_t bits = 0;
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                // the same test per 16-row tile: most passing tiles hold one
-                // pair, so the per-row loop below runs for one row tile
-                if constexpr (MT > 1) {
-                    const float mm = __builtin_fmaxf(__builtin_fmaxf(sc[m][0], sc[m][1]),
-                                                     __builtin_fmaxf(sc[m][2], sc[m][3]));
-                    if (__ballot(!(mm < thf)) == 0) continue;
-                }
+            for (int m = 0; m < MT; ++m)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
-                    // above the floor passes; at the floor the row part of
-                    // the key decides
-                    bool pass = sc[m][v] > thf;
-                    if (sc[m][v] == thf) pass = make_key(sc[m][v], row) >= thk[t];
-                    pass = pass && qi < a.nq && row < a.rows;
-                    const uint64_t hm = __ballot(pass);
-                    if (hm == 0) continue;
-                    const uint32_t n = __popcll(hm);
-                    if (cq_n + n > kCandQueue) flush();
+                for (int v = 0; v < 4; ++v) bits |= (sc[m][v] >= thf ? 1u : 0u) << (m * 4 + v);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (row0 + m * 16 + v >= a.rows) bits &= ~(1u << (m * 4 + v));
+            if (qi >= a.nq) bits = 0;
+            for (;;) {
+                const uint64_t hm = __ballot(bits != 0);
+                if (hm == 0) break;
+                const uint32_t n = __popcll(hm);
+                if (cq_n + n > kCandQueue) flush();
+                if (bits) {
+                    const uint32_t b = __builtin_ctz(bits);
+                    bits &= bits - 1;
+                    float sv = sc[0][0];
+#pragma unroll
+                    for (int i = 1; i < MT * 4; ++i) sv = b == (uint32_t)i ? sc[i / 4][i % 4] : sv;
                     const uint32_t pos = cq_n + __popcll(hm & __lanemask_lt());
-                    if (pass) {
-                        cq_key[pos] = make_key(sc[m][v], row);
-                        cq_qi[pos] = qi;
-                    }
-                    cq_n += n;
+                    cq_key[pos] = make_key(sv, row0 + (b >> 2) * 16 + (b & 3));
+                    cq_qi[pos] = qi;
                 }
+                cq_n += n;
             }
         }
     };
@@ -521,7 +713,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // chunk late, so one wave's epilogue overlaps the other's MFMAs.
     const bool late = E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
     v4f accp[MT][NT][E::NACC];
-    unsigned long long thkp[NT];
+    float thp[NT];
     for (uint32_t bt = blockIdx.x; bt < nblk; bt += gridDim.x) {
         // ---- stationary X: MT x MAXS units per lane (rows past the padded
         // count read the last row; their results are never written) ----
@@ -552,28 +744,53 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     bestq[m][v] = 0xFFFFFFFFu;
                 }
         }
-        float rs[MT][4];  // FP8 row scales of the lane's rows
+        // FP8: E8M0 scale of the lane's stationary row in each row tile
+        uint32_t sa[MT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                rs[m][v] = 1.f;
-                if constexpr (E::SCALED) {
-                    const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
-                    rs[m][v] = a.row_scale[row < a.rows_padded ? row : a.rows_padded - 1];
-                }
+        for (int m = 0; m < MT; ++m) {
+            sa[m] = 127u;
+            if constexpr (E::SCALED) {
+                const uint32_t row = bt * block_rows + wid * RW + m * 16 + l16;
+                sa[m] = e8m0_of(a.row_scale[row < a.rows_padded ? row : a.rows_padded - 1]);
             }
-        __syncthreads();  // previous block tile done with both buffers
-        fetch(ch0);
-        commit(0, ch0);
-        __syncthreads();
+        }
+        if constexpr (DMA) {
+            // the stationary rows (and scales) have landed: a wait the
+            // compiler sees, so it adds none for them inside the chunk loop
+            // (there it would also wait for the copies in flight)
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            // the previous tile left no copies in flight (its last chunk
+            // waited for all) and every wave is done reading its buffers
+            __syncthreads();
+            for (uint32_t j = 0; j + 1 < NB && ch0 + j < ch1; ++j) dma_chunk(ch0 + j, j);
+            wait_vm<0>();
+            __syncthreads();
+        } else {
+            __syncthreads();  // previous block tile done with both buffers
+            fetch(ch0);
+            commit(0, ch0);
+            __syncthreads();
+        }
         for (uint32_t ch = ch0; ch < ch1; ++ch) {
             const bool more = ch + 1 < ch1;
-            const uint32_t cur = (ch - ch0) & 1;
-            fetch(more ? ch + 1 : ch);
-            const unsigned long long *thc = thl(cur);
+            const uint32_t jj = ch - ch0;
+            const uint32_t cur = DMA ? jj % NB : (jj & 1);
+            if constexpr (!DMA) fetch(more ? ch + 1 : ch);
             const uint4 *src = qlds + cur * buf_units;
-            if (late && ch > ch0) epi0(accp, thkp, ch - 1, bt, rs);
+            // the chunk's thresholds (mode 0) and FP8 query scales into
+            // registers before the MFMAs: their LDS latency hides under them,
+            // and a late epilogue runs after other waves may have begun
+            // copying the next chunk into this buffer
+            float thv[NT];
+            uint32_t sbq[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                thv[t] = 0.f;
+                sbq[t] = 127u;
+                if (DMA && a.mode == 0) thv[t] = thfl(cur)[t * 16 + l16];
+                if constexpr (E::SCALED) sbq[t] = e8m0_of(qsl(cur)[t * 16 + l16]);
+            }
+            if (late && ch > ch0) epi0(accp, thp, ch - 1, bt);
             v4f acc[MT][NT][E::NACC];
 #pragma unroll
             for (int m = 0; m < MT; ++m)
@@ -615,27 +832,25 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                     for (int m = 0; m < MT; ++m) {
                         if constexpr (E::PAIR)
-                            E::mma2(xr[m][st], xr[m][st + 1], qc[0][t], qc[1][t], acc[m][t]);
+                            E::mma2(xr[m][st], xr[m][st + 1], qc[0][t], qc[1][t], sa[m], sbq[t],
+                                    acc[m][t]);
                         else
                             E::mma(xr[m][st], qc[0][t], acc[m][t]);
                     }
                 __builtin_amdgcn_sched_barrier(0);
             }
             if (!AM && a.mode == 0) {
-                unsigned long long thk[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t) thk[t] = thc[t * 16 + l16];
                 if (late) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
-                        thkp[t] = thk[t];
+                        thp[t] = thv[t];
 #pragma unroll
                         for (int m = 0; m < MT; ++m)
 #pragma unroll
                             for (int j = 0; j < E::NACC; ++j) accp[m][t][j] = acc[m][t][j];
                     }
                 } else {
-                    epi0(acc, thk, ch, bt, rs);
+                    epi0(acc, thv, ch, bt);
                 }
             }
             // ---- mode 1 and row argmax epilogue: lane holds rows 4*g+v of
@@ -644,17 +859,12 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             for (int t = 0; t < NT; ++t) {
                 if (!AM && a.mode == 0) break;
                 const uint32_t qi = ch * QC + t * 16 + l16;
-                float qs = 1.f;
-                if constexpr (E::SCALED) qs = qi < a.nq ? a.q_scale[qi] : 0.f;
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const uint32_t row0 = bt * block_rows + wid * RW + m * 16 + 4 * g;
                     float sc[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        sc[v] = E::total(acc[m][t], v);
-                        if constexpr (E::SCALED) sc[v] = sc[v] * (qs * rs[m][v]);
-                    }
+                    for (int v = 0; v < 4; ++v) sc[v] = E::total(acc[m][t], v);
                     if constexpr (AM) {
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
@@ -676,11 +886,22 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 }
             }
-            if (more) commit(cur ^ 1, ch + 1);
+            if constexpr (DMA) {
+                // chunk ch + NB - 1 into the buffer chunk ch - 1 used (every
+                // wave passed the barrier after reading it); then the copies
+                // of chunk ch + 1 must have landed before the barrier, while
+                // those of the chunks after it (ahead - 1 of them, each
+                // my_cnt copies, issued later in order) may stay in flight
+                if (ch + NB - 1 < ch1) dma_chunk(ch + NB - 1, (jj + NB - 1) % NB);
+                const uint32_t ahead = ch + NB - 1 < ch1 ? NB - 1 : ch1 - 1 - ch;
+                wait_vm_dyn(ahead >= 2 ? (ahead - 1) * my_cnt : 0);
+            } else {
+                if (more) commit(cur ^ 1, ch + 1);
+            }
             __syncthreads();
         }
         if constexpr (!AM) {
-            if (late) epi0(accp, thkp, ch1 - 1, bt, rs);
+            if (late) epi0(accp, thp, ch1 - 1, bt);
             if (cq_n) flush();
         }
         if constexpr (AM) {
@@ -723,12 +944,16 @@ __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__re
                                                                uint32_t n, uint32_t ld,
                                                                uint32_t kp,
                                                                const float *__restrict__ delta,
-                                                               unsigned long long *thresh) {
+                                                               unsigned long long *thresh,
+                                                               float *thresh_f) {
     __shared__ uint64_t slots[4][CW];
     __shared__ uint32_t cnt[4];
     const uint32_t q = blockIdx.x, lane = __lane_id(), wid = threadIdx.x >> 6;
     if (n < kp) {
-        if (threadIdx.x == 0) thresh[q] = 0;
+        if (threadIdx.x == 0) {
+            thresh[q] = 0;
+            thresh_f[q] = key_floor_sim(0);
+        }
         return;
     }
     WaveTopK tk;
@@ -764,6 +989,7 @@ __global__ __launch_bounds__(256) void sample_threshold_kernel(const float *__re
             // t + delta in the certificate, so no lowering is needed here.
             const float lowered = key_sim(slots[0][kp - 1]);
             thresh[q] = (unsigned long long)ordered_bits(lowered) << 32;
+            thresh_f[q] = key_floor_sim(thresh[q]);
         }
     }
 }
@@ -1127,9 +1353,12 @@ void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, f
 
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
 static void launch_mfma_split(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
-    // two query/threshold buffers, then (mode 0) the waves' candidate queues
-    const size_t lds = (2ull * (MAXS * 4 * 16 * NT + 8 * NT) +
-                        (AM ? 0 : NW * kCandQueue * 12 / 16)) * sizeof(uint4);
+    // the ring of query buffers, then the waves' candidate queues
+    constexpr size_t lds = mfma_lds_bytes<E, NW, MAXS, NT, AM>();
+    static_assert(lds <= 160 * 1024, "LDS of the MFMA scan");
+    if constexpr (!AM) {
+        if (!a.qimg) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the MFMA scan needs a chunk image");
+    }
     auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM, SPLIT>;
     // once per instantiation (also keeps the call out of graph captures);
     // serving executors launch from several threads, so the first call sets
@@ -1226,6 +1455,39 @@ static void launch_mfma_case(const MfmaScanArgs &a, int elem, uint32_t grid, hip
     return launch_mfma_t<F32E, 8, 16, ST, t.f32_nt>(a, grid, s);
 }
 
+void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_t &ru) {
+    if (variant < 0 || variant > 1 || (uint32_t)variant >= mfma_variants(elem, dch)) variant = 0;
+    const int i = mfma_tile_index(dch);
+    if (i < 0)
+        throw Error(NNDEX_ERR_UNSUPPORTED, "padded dims above 512 (f32) / 1024 (bf16) / 2048 "
+                                           "(fp8) are not supported by the MFMA scan");
+    const MfmaTiles &t = kMfmaTiles[i];
+    int nt;
+    if (elem == kElemFP8)
+        nt = variant ? t.f8_nt1 : t.f8_nt;
+    else if (elem == kElemBF16)
+        nt = variant ? t.bf_nt1 : t.bf_nt;
+    else
+        nt = variant ? t.f32_nt1 : t.f32_nt;
+    qc = 16u * (uint32_t)nt;
+    ru = 4u * t.steps;
+}
+
+void launch_chunk_image(const uint4 *q, uint32_t q_stride, uint32_t nq, uint32_t dch, uint32_t qc,
+                        uint32_t ru, uint4 *img, float *thresh_pad, float *scale_pad,
+                        hipStream_t s) {
+    if (nq == 0) return;
+    const uint32_t nch = (nq + qc - 1) / qc;
+    const size_t total = (size_t)nch * qc * ru;
+    if (total > mfma_image_units(nq))
+        throw Error(NNDEX_ERR_INVALID_ARGUMENT, "chunk image larger than its buffer");
+    uint32_t grid = (uint32_t)((total + 255) / 256);
+    if (grid > 2048) grid = 2048;
+    hipLaunchKernelGGL(chunk_image_kernel, dim3(grid), dim3(256), 0, s, q, q_stride, nq, dch, qc,
+                       ru, nch, img, thresh_pad, scale_pad);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
 void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s,
                       int variant) {
     if (variant < 0 || variant > 1 || (uint32_t)variant >= mfma_variants(elem, a.dch))
@@ -1253,19 +1515,19 @@ uint32_t cw_for(uint32_t k) {
 
 void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
                              uint32_t kp, const float *delta, unsigned long long *thresh,
-                             hipStream_t s) {
+                             float *thresh_f, hipStream_t s) {
     switch (cw_for(kp)) {
     case 128:
         hipLaunchKernelGGL(sample_threshold_kernel<128>, dim3(nq), dim3(256), 0, s, scores, n, ld,
-                           kp, delta, thresh);
+                           kp, delta, thresh, thresh_f);
         break;
     case 256:
         hipLaunchKernelGGL(sample_threshold_kernel<256>, dim3(nq), dim3(256), 0, s, scores, n, ld,
-                           kp, delta, thresh);
+                           kp, delta, thresh, thresh_f);
         break;
     case 512:
         hipLaunchKernelGGL(sample_threshold_kernel<512>, dim3(nq), dim3(256), 0, s, scores, n, ld,
-                           kp, delta, thresh);
+                           kp, delta, thresh, thresh_f);
         break;
     default:
         throw Error(NNDEX_ERR_UNSUPPORTED, "over-fetch above 448 is not supported");

@@ -17,11 +17,12 @@ struct MfmaScanArgs {
     uint32_t dch;              // 16-byte units per row
     const uint4 *q;            // queries, row-major, q_stride units per query
     uint32_t q_stride;
-    const float *row_scale;    // fp8: per-row scale [rows_padded]
-    const float *q_scale;      // fp8: per-query scale [nq]
+    const float *row_scale;    // fp8: per-row power-of-two scale [rows_padded]
+    const float *q_scale;      // fp8: per-query power-of-two scale [nq]
     uint32_t nq;
     uint32_t mode;             // 0: append candidates, 1: write scores, 2: row argmax
-    const unsigned long long *thresh;  // [nq] key thresholds (mode 0)
+    const unsigned long long *thresh;  // [nq] key thresholds (register staging)
+    const float *thresh_f;     // [nq] their float floors (mode 0, padded: see qimg)
     uint64_t *cand;            // [nq][capq] candidate keys (mode 0)
     uint32_t *cand_count;      // [nq] (mode 0; may exceed capq on overflow)
     uint32_t capq;
@@ -37,6 +38,12 @@ struct MfmaScanArgs {
     // query splits (grid y): each block row scans its share of the query
     // chunks (0 or 1: one share)
     uint32_t qsplit;
+    // modes 0 and 1: the batch's chunk image (launch_chunk_image for this
+    // element type, width and variant), which the scan copies into LDS;
+    // `thresh_f` (mode 0) and `q_scale` (FP8) are then read a whole chunk at
+    // a time, so both hold nq rounded up to the chunk's query count (the
+    // image's padding entries set by launch_chunk_image)
+    const uint4 *qimg;
 };
 
 struct RerankArgs {
@@ -56,7 +63,8 @@ struct RerankArgs {
 };
 
 void launch_to_bf16(const float4 *x, size_t rows_padded, uint32_t d8, uint4 *y, hipStream_t s);
-// FP8-E4M3 (OCP) copy with a per-row scale s_r = max|x_r| / 448: tiled units of
+// FP8-E4M3 (OCP) copy with a per-row power-of-two scale s_r >= max|x_r| / 448
+// (the least such; an E8M0 block scale of the MFMA): tiled units of
 // 16 codes (d16 = ceil(d4 / 4) units per row), scales [rows_padded], and the
 // largest per-row norm of |s_r x~_r - x_r| (f64 bits, atomic max into *err_sq).
 void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, float *scale,
@@ -75,9 +83,23 @@ uint32_t mfma_variants(int elem, uint32_t dch);
 uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax = false, int variant = 0);
 void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s,
                       int variant = 0);
+// Chunk shape of the candidate / score scan for (element type, units per row,
+// variant): queries per chunk and unit rows per query (4 * steps >= dch).
+void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_t &ru);
+// Largest chunk image of any shape for nq queries, in 16-byte units.
+inline size_t mfma_image_units(size_t nq) { return (nq + 64) * 128; }
+// Writes the chunk image of nq row-major queries (q_stride units apart, dch
+// valid units each) for the scan shape (qc, ru), and sets the padding
+// entries of thresh_pad (+inf) and scale_pad (0) up to the last chunk;
+// either may be null.
+void launch_chunk_image(const uint4 *q, uint32_t q_stride, uint32_t nq, uint32_t dch, uint32_t qc,
+                        uint32_t ru, uint4 *img, float *thresh_pad, float *scale_pad,
+                        hipStream_t s);
+// Per query the key threshold (the kp-th best sample score, row part zero)
+// and its float floor (key_floor_sim), which the candidate scan compares.
 void launch_sample_threshold(const float *scores, uint32_t nq, uint32_t n, uint32_t ld,
                              uint32_t kp, const float *delta, unsigned long long *thresh,
-                             hipStream_t s);
+                             float *thresh_f, hipStream_t s);
 void launch_rerank(const RerankArgs &a, uint32_t nq, hipStream_t s);
 void launch_rerank_full(const RerankArgs &a, uint32_t nq, hipStream_t s);
 // Tiled FP32 rows -> row-major (d4 float4 per row): the reranks read whole

@@ -52,7 +52,8 @@ static void regrow(T *&p, size_t n) {
 }
 
 Workspace::~Workspace() {
-    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow, (void *)delta, (void *)qscale,
+    for (void *p : {(void *)part, (void *)part_count, (void *)theta, (void *)ticket, (void *)qlow,
+                    (void *)qimg, (void *)thresh_f, (void *)delta, (void *)qscale,
                     (void *)scores, (void *)thresh, (void *)cand, (void *)cand_count,
                     (void *)flags, (void *)ivf_probe, (void *)ivf_probe_sim, (void *)ivf_part,
                     (void *)xflags, (void *)ivf_pair_u32, (void *)ivf_list_u32, ivf_sort_tmp,
@@ -121,9 +122,13 @@ void Workspace::reserve_batch(size_t nq, size_t pdims, size_t rows_s, size_t cap
     if (nq > b_queries || nq * rows_s > b_scores || nq * capq > b_cand) ++generation;
     if (nq > b_queries) {
         regrow(qlow, nq * pdims / 8);
+        regrow(qimg, mfma_image_units(nq));
         regrow(delta, nq);
-        regrow(thresh, nq);
-        regrow(qscale, nq);
+        // thresholds and FP8 scales are copied a whole query chunk (<= 64)
+        // at a time by the MFMA scan
+        regrow(thresh, nq + 64);
+        regrow(thresh_f, nq + 64);
+        regrow(qscale, nq + 64);
         regrow(cand_count, nq);
         regrow(flags, nq);
         b_queries = nq;
@@ -410,8 +415,16 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
                             ws.qlow, ws.qscale, xmax_sq_,
                             bf16 ? xerr_sq_ + 1 : (fp8 ? xerr8_sq_ : xmax_sq_), ws.delta, s);
         const uint4 *qu = p != Precision::F32 ? ws.qlow : reinterpret_cast<const uint4 *>(qf);
+        const int variant = mfma_variant_[elem].load(std::memory_order_relaxed);
+        // the chunk image both scans copy their query chunks from (its
+        // padding thresholds never pass; padding scales are zero)
+        uint32_t qc = 0, ru = 0;
+        mfma_chunk_shape(elem, dch, variant, qc, ru);
+        launch_chunk_image(qu, dch, static_cast<uint32_t>(n), dch, qc, ru, ws.qimg, ws.thresh_f,
+                           fp8 ? ws.qscale : nullptr, s);
         // stage 1: sample scores and thresholds
         MfmaScanArgs a{};
+        a.qimg = ws.qimg;
         a.x = bf16 ? smp.bf16 : (fp8 ? smp.fp8 : reinterpret_cast<const uint4 *>(smp.f32));
         a.row_scale = smp.fp8_scale;
         a.q_scale = ws.qscale;
@@ -424,7 +437,6 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.mode = 1;
         a.out = ws.scores;
         a.ldo = static_cast<uint32_t>(sample_ld(smp.rows));
-        const int variant = mfma_variant_[elem].load(std::memory_order_relaxed);
         const uint32_t br = mfma_block_rows(elem, dch, false, variant);
         // one 8-wave block per CU (register-bound); a scan with fewer row
         // tiles than CUs also splits the queries over blocks (grid y),
@@ -449,7 +461,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         };
         launch_mfma_scan(a, elem, grid_for(a.rows_padded), s, variant);
         launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, a.ldo,
-                                static_cast<uint32_t>(plan.jt), ws.delta, ws.thresh, s);
+                                static_cast<uint32_t>(plan.jt), ws.delta, ws.thresh, ws.thresh_f, s);
         // stage 2: filter scan over every row
         launch_zero_u32(ws.cand_count, n, s);
         a.x = bf16 ? xbf16_ : (fp8 ? xfp8_ : reinterpret_cast<const uint4 *>(x_));
@@ -458,6 +470,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.rows_padded = static_cast<uint32_t>(padded_rows(rows_));
         a.mode = 0;
         a.thresh = ws.thresh;
+        a.thresh_f = ws.thresh_f;
         a.cand = ws.cand;
         a.cand_count = ws.cand_count;
         a.capq = static_cast<uint32_t>(capq);
@@ -540,7 +553,8 @@ BatchPlan GpuIndex::plan_batch(size_t k, Precision p, size_t kp) const {
 
 size_t BatchPlan::workspace_bytes(size_t nq, size_t pdims) const {
     const size_t n = nq < group ? nq : group;
-    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 4 + 1) + n * sample_ld(rows_s) * 4 + n * capq * 8;
+    return n * (pdims / 8 * 16 + 4 + 8 + 4 + 4 + 1) + (n + 64) * (8 + 4 + 4) +
+           mfma_image_units(n) * 16 + n * sample_ld(rows_s) * 4 + n * capq * 8;
 }
 
 bool GpuIndex::exact_batch_precision(Precision &p) const {

@@ -49,9 +49,11 @@ struct Workspace {
     // batched MFMA path
     size_t b_queries = 0, b_scores = 0, b_cand = 0;
     uint4 *qlow = nullptr;
+    uint4 *qimg = nullptr;  // chunk image of the MFMA scan's queries
     float *delta = nullptr;
     float *scores = nullptr;
     unsigned long long *thresh = nullptr;
+    float *thresh_f = nullptr;  // the thresholds' float floors
     float *qscale = nullptr;
     uint64_t *cand = nullptr;
     uint32_t *cand_count = nullptr;

@@ -136,3 +136,157 @@ def test_ivf_scan_indices_stay_in_bounds(orc, rows, dims, k, nprobe, nq):
     if large:
         # the pre-fix slot leaves the key array (by about 2^32 keys)
         assert worst_old >= length and worst_old >= U32
+
+
+# ---------------------------------------------------------------------------
+# The MFMA scan's ring of LDS query buffers (nndex_amd/csrc/batch.hip,
+# mfma_scan_kernel with DMA staging): per block tile, a prologue copies
+# chunks ch0 .. ch0 + NB - 2 into buffers 0 .. NB - 2 and waits for them;
+# iteration j (chunk ch0 + j, buffer j % NB) then issues chunk j + NB - 1
+# into buffer (j + NB - 1) % NB, waits until at most (ahead - 1) * my_cnt of
+# its copies are in flight, and meets the others at a barrier.  The replay
+# keeps every wave's in-flight copies in issue order, lets a copy land only
+# when a wait forces it (the latest it may), and checks that a buffer is
+# never read before its chunk has landed, nor written while another chunk
+# is read from it; and that each copy stays inside the image, threshold and
+# scale arrays and the LDS ring.
+# ---------------------------------------------------------------------------
+
+def _nbuf(maxs, nt, scaled, nw=8, queue=True):
+    buf = (maxs * 4 * 16 * nt + 8 * nt + (4 * nt if scaled else 0)) * 16
+    q = nw * 128 * 12 if queue else 0
+    lds = 160 * 1024
+    return 4 if 4 * buf + q <= lds else (3 if 3 * buf + q <= lds else 2)
+
+
+def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2):
+    qc = 16 * nt
+    ru = 4 * maxs
+    q_units = qc * ru
+    tu, su = qc // 2, (qc // 4 if scaled else 0)
+    buf_units = q_units + tu + su
+    nb = _nbuf(maxs, nt, scaled, nw)
+    assert (nb * buf_units + nw * 128 * 12 // 16) * 16 <= 160 * 1024
+    ni = nt * maxs
+    nch_all = -(-nq // qc)
+    image_units = nch_all * qc * ru
+    assert image_units <= (nq + 64) * 128      # mfma_image_units
+    thresh_len = nq + 64                        # Workspace::reserve_batch
+    stage_th = mode == 0
+
+    def copies(w, ch, buf):
+        """The copies wave w issues for chunk ch: (lds lo, lds hi, src lo, src hi, array)."""
+        out = []
+        for i in range(w, ni, nw):
+            out.append((buf * buf_units + i * 64, buf * buf_units + i * 64 + 64,
+                        ch * q_units + i * 64, ch * q_units + i * 64 + 64, "img"))
+        if stage_th and w == ni % nw:
+            out.append((buf * buf_units + q_units, buf * buf_units + q_units + tu,
+                        ch * qc, ch * qc + qc, "thr"))
+        if scaled and w == (ni + 1) % nw:
+            out.append((buf * buf_units + q_units + tu, buf * buf_units + q_units + tu + su,
+                        ch * qc, ch * qc + qc, "scale"))
+        for lo, hi, s0, s1, arr in out:
+            assert 0 <= lo < hi <= nb * buf_units
+            assert hi - lo <= 64  # one copy instruction: at most 64 lanes x 16 bytes
+            if arr == "img":
+                assert 0 <= s0 and s1 <= image_units
+            else:
+                assert s1 <= thresh_len
+        return out
+
+    for split in range(qsplit):
+        per = -(-nch_all // qsplit)
+        ch0, ch1 = split * per, min(split * per + per, nch_all)
+        if ch0 >= ch1:
+            continue
+        for _ in range(tiles):
+            inflight = [[] for _ in range(nw)]   # per wave: (buf, chunk) in issue order
+            landed = {}                          # buf -> chunk whose copies all landed
+            pending = {}                         # buf -> set of (wave, chunk) in flight
+
+            def issue(w, ch, buf):
+                cnt = 0
+                for c in copies(w, ch, buf):
+                    inflight[w].append((buf, ch))
+                    pending.setdefault(buf, set()).add((w, ch))
+                    cnt += 1
+                return cnt
+
+            def wait(w, n):
+                while len(inflight[w]) > n:
+                    buf, ch = inflight[w].pop(0)
+                    if all(x != (buf, ch) for x in inflight[w]):
+                        pending[buf].discard((w, ch))
+
+            my_cnt = [len(copies(w, ch0, 0)) for w in range(nw)]
+            for j in range(nb - 1):
+                if ch0 + j < ch1:
+                    for w in range(nw):
+                        issue(w, ch0 + j, j)
+            for w in range(nw):
+                wait(w, 0)
+            for j in range(nb - 1):
+                if ch0 + j < ch1:
+                    assert not pending.get(j)
+                    landed[j] = ch0 + j
+            for ch in range(ch0, ch1):
+                jj = ch - ch0
+                cur = jj % nb
+                # the barrier before this iteration: its chunk landed, and no
+                # copy into its buffer is in flight
+                assert landed.get(cur) == ch, (ch, cur, landed)
+                assert not pending.get(cur), (ch, cur, pending.get(cur))
+                for w in range(nw):
+                    if ch + nb - 1 < ch1:
+                        tgt = (jj + nb - 1) % nb
+                        assert tgt != cur
+                        landed.pop(tgt, None)
+                        assert issue(w, ch + nb - 1, tgt) == my_cnt[w]
+                    ahead = nb - 1 if ch + nb - 1 < ch1 else ch1 - 1 - ch
+                    wait(w, (ahead - 1) * my_cnt[w] if ahead >= 2 else 0)
+                # barrier: the next chunk's copies have landed in every wave
+                nxt = ch + 1
+                if nxt < ch1:
+                    b = (jj + 1) % nb
+                    assert not pending.get(b), (nxt, b, pending.get(b))
+                    landed[b] = nxt
+            assert all(not inflight[w] for w in range(nw))
+
+
+@pytest.mark.parametrize("nq,maxs,nt,mode,scaled,qsplit", [
+    (2048, 12, 2, 0, False, 1),   # BF16 384 dims: 4 buffers
+    (2048, 12, 2, 1, False, 2),   # its sample scan, queries split over 2 block rows
+    (1024, 24, 2, 0, False, 1),   # BF16 768 dims: 3 buffers
+    (1024, 24, 1, 0, False, 1),
+    (1024, 12, 2, 0, True, 1),    # FP8 768 dims: thresholds and scales
+    (1024, 32, 1, 1, True, 3),
+    (17, 4, 4, 0, False, 1),      # one partial chunk
+    (100, 8, 2, 0, False, 1),     # tail shorter than the ring
+    (33, 2, 4, 1, False, 1),
+    (4096, 32, 1, 0, False, 8),
+])
+def test_mfma_scan_ring_schedule(nq, maxs, nt, mode, scaled, qsplit):
+    ring_replay(nq, maxs, nt, mode, scaled, qsplit=qsplit)
+
+
+def test_chunk_image_inverts_the_lds_swizzle():
+    """chunk_image_kernel writes query n of unit row c at slot
+    (n & ~15) | ((n ^ c) & 15); the scan reads slot (l16 ^ (4 (S & 3) + g)) &
+    15 of row 4 S + g for the query of lane group position l16: both must
+    name the same query."""
+    for qc in (16, 32, 64):
+        for c in range(128):
+            slots = [(n & ~15) | ((n ^ c) & 15) for n in range(qc)]
+            assert sorted(slots) == list(range(qc))
+            for slot in range(qc):
+                n = (slot & ~15) | ((slot ^ c) & 15)   # the image kernel's inverse
+                assert slots[n] == slot
+    for s_ in range(32):
+        for g in range(4):
+            u = 4 * s_ + g
+            for t in range(4):
+                for l16 in range(16):
+                    n = t * 16 + l16
+                    read_slot = t * 16 + ((l16 ^ (4 * (s_ & 3) + g)) & 15)
+                    assert read_slot == (n & ~15) | ((n ^ u) & 15)
