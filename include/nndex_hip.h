@@ -416,6 +416,15 @@ nndex_status nndex_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Plain streaming copy kernel (bandwidth calibration). */
 nndex_status nndex_device_copy(void *device_dst, const void *device_src, size_t bytes,
                                void *stream);
+/* Diagnostic with no counterpart in the reference, for tests of the batched
+ * path's certificate: one wavefront runs `calls` chained MFMAs of the
+ * candidate scan's element form `elem` (0 FP32 16x16x4, 1 BF16 16x16x32,
+ * 2 FP8-E4M3 16x16x128 with unit scales) over host operands laid out per
+ * call and lane (FP32: one float; BF16: 8 values; FP8: 32 codes; lane l
+ * holds row / column l % 16 and K block l / 16), and writes the 16 x 16 FP32
+ * result row-major to host `out`.  Synchronous on the current device. */
+nndex_status nndex_debug_mfma_tile(int32_t elem, const void *a, const void *b, uint32_t calls,
+                                   float *out);
 
 #ifdef __cplusplus
 }

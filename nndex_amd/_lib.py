@@ -192,6 +192,7 @@ SIGNATURES = {
     "nndex_graph_launch": (_ST, [_P, _P]),
     "nndex_graph_free": (None, [_P]),
     "nndex_device_copy": (_ST, [_P, _P, _SZ, _P]),
+    "nndex_debug_mfma_tile": (_ST, [_I32, _P, _P, ctypes.c_uint32, _P]),
     "nndex_new": (_ST, [_FP, _SZ, _SZ, _SZ, ctypes.POINTER(IndexOptions), ctypes.POINTER(_P)]),
     "nndex_free": (None, [_P]),
     "nndex_backend_of": (_I32, [_P]),

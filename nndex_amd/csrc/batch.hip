@@ -1351,6 +1351,43 @@ void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, f
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
+// One wavefront's chain of `calls` MFMAs in the scan's element form over
+// caller operands (lane-major per call: FP32 one float per lane, BF16 eight
+// values, FP8 32 codes with unit E8M0 scales), the 16 x 16 FP32 result
+// row-major in `out` (row 4 (lane / 16) + v, column lane % 16).  Tests check
+// each form's accumulation error against the bound the certificate assumes.
+__global__ __launch_bounds__(64) void mfma_tile_kernel(int elem, const uint4 *__restrict__ a,
+                                                       const uint4 *__restrict__ b,
+                                                       uint32_t calls, float *__restrict__ out) {
+    const uint32_t l = threadIdx.x;
+    v4f c[4] = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f},
+                v4f{0.f, 0.f, 0.f, 0.f}};
+    for (uint32_t k = 0; k < calls; ++k) {
+        if (elem == kElemF32) {
+            const float *af = reinterpret_cast<const float *>(a), *bf = reinterpret_cast<const float *>(b);
+            c[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[k * 64 + l], bf[k * 64 + l], c[0], 0, 0, 0);
+        } else if (elem == kElemBF16) {
+            v4f one[1] = {c[0]};
+            BF16E::mma(a[k * 64 + l], b[k * 64 + l], one);
+            c[0] = one[0];
+        } else {
+            v4f one[1] = {c[0]};
+            FP8E::mma2(a[(k * 64 + l) * 2], a[(k * 64 + l) * 2 + 1], b[(k * 64 + l) * 2],
+                       b[(k * 64 + l) * 2 + 1], 127u, 127u, one);
+            c[0] = one[0];
+        }
+    }
+    for (int v = 0; v < 4; ++v) out[(4 * (l / 16) + v) * 16 + l % 16] = c[0][v];
+}
+
+void launch_mfma_tile(int elem, const void *a, const void *b, uint32_t calls, float *out,
+                      hipStream_t s) {
+    if (elem < kElemF32 || elem > kElemFP8) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "element form");
+    hipLaunchKernelGGL(mfma_tile_kernel, dim3(1), dim3(64), 0, s, elem,
+                       static_cast<const uint4 *>(a), static_cast<const uint4 *>(b), calls, out);
+    NNDEX_HIP_CHECK(hipGetLastError());
+}
+
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
 static void launch_mfma_split(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     // the ring of query buffers, then the waves' candidate queues

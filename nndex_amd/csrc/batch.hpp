@@ -108,6 +108,11 @@ void launch_tiled_to_rows(const float4 *x, uint32_t rows, uint32_t d4, float4 *y
 void launch_gather_rows(const float4 *x, uint32_t d4, uint32_t stride, uint32_t rows_s,
                         float4 *dst, hipStream_t s);
 void launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+// One wavefront's chain of `calls` MFMAs of element form `elem` over device
+// operands (per call and lane: 4 bytes FP32, 16 bytes BF16, 32 bytes FP8),
+// the 16 x 16 FP32 result row-major into out (device).
+void launch_mfma_tile(int elem, const void *a, const void *b, uint32_t calls, float *out,
+                      hipStream_t s);
 // Row-major FP32 rows -> row-major BF16 (16-byte units of 8), and the largest
 // per-row norm of the rounding error (f64 bits, atomic max into *err_sq).
 void launch_rows_to_bf16(const float *rows, uint32_t n, uint32_t pdims, uint4 *out,

@@ -1835,6 +1835,24 @@ nndex_status nndex_device_copy(void *device_dst, const void *device_src, size_t 
     });
 }
 
+nndex_status nndex_debug_mfma_tile(int32_t elem, const void *a, const void *b, uint32_t calls,
+                                   float *out) {
+    return guard([&] {
+        if (!a || !b || !out) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");
+        if (calls == 0 || calls > 4096) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "calls in [1, 4096]");
+        const size_t per = elem == kElemF32 ? 4 : (elem == kElemBF16 ? 16 : 32);
+        const size_t bytes = per * 64 * calls;
+        DeviceBuffer da, db, dout;
+        da.reserve(bytes);
+        db.reserve(bytes);
+        dout.reserve(256 * sizeof(float));
+        NNDEX_HIP_CHECK(hipMemcpy(da.ptr, a, bytes, hipMemcpyHostToDevice));
+        NNDEX_HIP_CHECK(hipMemcpy(db.ptr, b, bytes, hipMemcpyHostToDevice));
+        launch_mfma_tile(elem, da.ptr, db.ptr, calls, static_cast<float *>(dout.ptr), nullptr);
+        NNDEX_HIP_CHECK(hipMemcpy(out, dout.ptr, 256 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
 nndex_status nndex_device_count(int32_t *count) {
     return guard([&] {
         if (!count) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "null pointer");

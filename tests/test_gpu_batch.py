@@ -272,3 +272,67 @@ def test_wide_rows_tile_variants(dims):
                             assert s == exact[(i, int(r))], (v, i, int(r))
     check(L.nndex_gpu_index_set_batch_tiles(b.h, 1, 0))
     check(L.nndex_gpu_index_set_batch_tiles(b.h, 2, 0))
+
+
+def _fp8_values():
+    """The 256 OCP E4M3 code values (NaN codes 0x7F / 0xFF as nan)."""
+    c = np.arange(256)
+    e, m = (c >> 3) & 15, c & 7
+    v = np.where(e == 0, m * 2.0 ** -9, (1 + m / 8.0) * 2.0 ** (e - 7.0))
+    v = np.where((c & 0x7F) == 0x7F, np.nan, v)
+    return np.where(c & 0x80, -v, v)
+
+
+@pytest.mark.parametrize("elem", [0, 1, 2])
+def test_mfma_accumulation_within_certificate_bounds(elem):
+    """The certificate bounds |approximate - exact| of every MFMA score by
+    gamma_K sum|a b| for the FP32 and BF16 forms (an FP32 sum) and by
+    kFp8MfmaRelErr = 2^-8 of sum|a b| per 128-product instruction, plus FP32
+    accumulation over the chain, for FP8 (whose MFMA does not sum within
+    FP32's bound).  Random operands of narrow and wide exponent ranges, one
+    instruction and chains of 12, against exact float64 sums."""
+    L = _lib.lib()
+    rng = np.random.default_rng(100 + elem)
+    u = 2.0 ** -24
+    f8 = _fp8_values()
+    worst = 0.0
+    for trial in range(24):
+        calls = 1 if trial % 2 == 0 else 12
+        wide = trial % 4 >= 2
+        if elem == 0:
+            a = (rng.uniform(-1, 1, (calls, 64)) * (2.0 ** rng.integers(-20, 20, (calls, 64)) if wide else 1)).astype(np.float32)
+            b = (rng.uniform(-1, 1, (calls, 64)) * (2.0 ** rng.integers(-20, 20, (calls, 64)) if wide else 1)).astype(np.float32)
+            av, bv = a.astype(np.float64)[..., None], b.astype(np.float64)[..., None]
+            per_call = 4
+        elif elem == 1:
+            fa = (rng.uniform(-1, 1, (calls, 64, 8)) * (2.0 ** rng.integers(-20, 20, (calls, 64, 8)) if wide else 1)).astype(np.float32)
+            fb = (rng.uniform(-1, 1, (calls, 64, 8)) * (2.0 ** rng.integers(-20, 20, (calls, 64, 8)) if wide else 1)).astype(np.float32)
+            a = (fa.view(np.uint32) >> 16).astype(np.uint16)
+            b = (fb.view(np.uint32) >> 16).astype(np.uint16)
+            av = (a.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+            bv = (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+            per_call = 32
+        else:
+            lo = 0 if wide else 0x30
+            hi = 0x7E if wide else 0x60
+            a = (rng.integers(lo, hi, (calls, 64, 32)) | (rng.integers(0, 2, (calls, 64, 32)) << 7)).astype(np.uint8)
+            b = (rng.integers(lo, hi, (calls, 64, 32)) | (rng.integers(0, 2, (calls, 64, 32)) << 7)).astype(np.uint8)
+            av, bv = f8[a], f8[b]
+            per_call = 128
+        out = np.zeros(256, np.float32)
+        check(L.nndex_debug_mfma_tile(elem, a.ctypes.data, b.ctypes.data, calls, out.ctypes.data))
+        got = out.reshape(16, 16).astype(np.float64)
+        # lane l holds row / column l % 16 and K block l / 16
+        A = av.reshape(calls, 4, 16, -1).transpose(2, 0, 1, 3).reshape(16, -1)
+        B = bv.reshape(calls, 4, 16, -1).transpose(2, 0, 1, 3).reshape(16, -1)
+        exact = A @ B.T
+        absum = np.abs(A) @ np.abs(B).T
+        K = per_call * calls
+        if elem == 2:
+            bound = (2.0 ** -8 + (calls + 1) * u / (1 - (calls + 1) * u)) * absum
+        else:
+            bound = K * u / (1 - K * u) * absum
+        err = np.abs(got - exact)
+        assert np.all(err <= bound), (trial, float((err / np.maximum(absum, 1e-300)).max()))
+        worst = max(worst, float((err / np.maximum(absum, 1e-300)).max()))
+    print(f"form {elem}: largest |err| / sum|ab| {worst:.3e}", flush=True)
