@@ -76,9 +76,10 @@ struct FP8E {
     // one 16x16x128 f8f6f4 MFMA (both operands OCP E4M3) per two 16-byte
     // steps: lane group g supplies the 32 codes of its units 4s+g and
     // 4(s+1)+g, rows and queries alike, so the K order is the same on both
-    // sides.  Row and query scales are powers of two, applied by the MFMA
-    // itself as E8M0 block scales (sa: the lane's row, sb: the lane's query;
-    // the same for every K block), so the scores come out scaled.
+    // sides.  The exponents of the row and query scales are applied by the
+    // MFMA itself as E8M0 block scales (sa: the lane's row, sb: the lane's
+    // query; the same for every K block); the epilogue multiplies by the
+    // scales' mantissas.
     static constexpr int NACC = 1;
     static constexpr bool SCALED = true, PAIR = true, SKEW = true;
     static __device__ inline void mma(uint4, uint4, v4f (&)[NACC]) {}
@@ -123,19 +124,15 @@ __device__ inline float fp8_e4m3_value(uint32_t c) {
     return (c & 0x80u) ? -v : v;
 }
 
-// Scale of an FP8 row or query: the least power of two p >= max|v| / 448, so
-// every code lies within the E4M3 range and p is an E8M0 block scale (E4M3
-// is floating point: a power-of-two scale costs no precision); kept at or
-// above 2^-126 (a normal float), 0 for an all-zero vector.
-__host__ __device__ inline float fp8_pow2_scale(float mx) {
+// Scale of an FP8 row or query: s = max|v| / 448, so the largest element is
+// the largest code, 448, exactly (the largest elements carry most of a
+// score; a power-of-two scale that leaves them off a code doubles the
+// largest score error on anisotropic data), kept at or above 2^-126 (a normal
+// float: its exponent is the MFMA's E8M0 block scale), 0 for a zero vector.
+__host__ __device__ inline float fp8_scale(float mx) {
     if (!(mx > 0.f)) return 0.f;
     const float r = mx / 448.0f;
-    const uint32_t b = __builtin_bit_cast(uint32_t, r);
-    uint32_t e = (b >> 23) & 0xFFu;                  // r = 1.m x 2^(e - 127)
-    if (e == 0) return 0x1p-126f;                    // subnormal r: p = 2^-126
-    if ((b & 0x7FFFFFu) != 0) ++e;                   // not a power of two: round up
-    if (e > 254) e = 254;
-    return __builtin_bit_cast(float, e << 23);
+    return r < 0x1p-126f ? 0x1p-126f : r;
 }
 // Bound on |result - exact| / sum |a_k b_k| of one FP8 MFMA over its 128
 // products (v_mfma_scale_f32_16x16x128_f8f6f4).  Its accumulation is not
@@ -144,10 +141,16 @@ __host__ __device__ inline float fp8_pow2_scale(float mx) {
 // used is 2^-8, 28x the largest measured.  The FP32 forms (16x16x4f32) and
 // BF16 (16x16x32) measured within their FP32 bounds.
 constexpr double kFp8MfmaRelErr = 0x1p-8;
-// E8M0 byte of a power-of-two scale (127 for a zero scale: its codes are 0).
-__device__ inline uint32_t e8m0_of(float p) {
-    const uint32_t e = (__builtin_bit_cast(uint32_t, p) >> 23) & 0xFFu;
+// A scale s = 2^e * f, f in [1, 2): the E8M0 byte of 2^e, which the MFMA
+// applies as a block scale (127 for a zero scale: its codes are 0), and the
+// factor f, which the scan's epilogue applies.
+__device__ inline uint32_t e8m0_of(float s) {
+    const uint32_t e = (__builtin_bit_cast(uint32_t, s) >> 23) & 0xFFu;
     return e ? e : 127u;
+}
+__device__ inline float mant_of(float s) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, s);
+    return (b & 0x7F800000u) ? __builtin_bit_cast(float, (b & 0x007FFFFFu) | 0x3F800000u) : 1.f;
 }
 
 // ---------------------------------------------------------------------------
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(256) void to_fp8_kernel(const float4 *__restrict__ 
                                                      __builtin_fmaxf(__builtin_fabsf(v.z),
                                                                     __builtin_fabsf(v.w))));
         }
-        const float sc = fp8_pow2_scale(mx);
+        const float sc = fp8_scale(mx);
         const uint32_t d16 = (d4 + 3) / 4;
         uint4 *o = y + (size_t)(row / 64) * d16 * 64 + row % 64;
         for (uint32_t u = 0; u < d16; ++u) {
@@ -359,11 +362,12 @@ __global__ __launch_bounds__(256) void max_norm_kernel(const float4 *__restrict_
 
 // ---------------------------------------------------------------------------
 // Query preparation: low-precision copy (row-major, 16-byte units; FP8 with a
-// per-query power-of-two scale s_q >= max|q| / 448) and the per-query error
-// bound delta of |approximate - exact| over any row, with x~, q~ the
-// dequantised copies (x~ = x, q~ = q for FP32):
+// per-query scale s_q = max|q| / 448) and the per-query error bound delta of
+// |approximate - exact| over any row, with x~, q~ the dequantised copies
+// (x~ = x, q~ = q for FP32):
 //   exact side   gamma_n |x| |q|                 (four-lane FMA chain)
-//   approx side  gamma_n |x~| |q~|  (FP8: kFp8MfmaRelErr + gamma_(n/128+1))
+//   approx side  gamma_n |x~| |q~|  (FP8: kFp8MfmaRelErr + gamma_(n/128+1)
+//                                     + 3u for the scale mantissas)
 //   rounding     |x~| |q~ - q| + |x~ - x| |q|
 // with |x| <= xmax (largest row norm), |x~ - x| <= xe (largest row rounding
 // error, measured when the copy was made) and |x~| <= xmax + xe.
@@ -391,7 +395,7 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
                 smax[threadIdx.x] = __builtin_fmaxf(smax[threadIdx.x], smax[threadIdx.x + st]);
             __syncthreads();
         }
-        sq = fp8_pow2_scale(smax[0]);
+        sq = fp8_scale(smax[0]);
         if (threadIdx.x == 0) q_scale[qi] = sq;
     }
     double nq2 = 0, nl2 = 0, nd2 = 0;
@@ -456,7 +460,9 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
         // its error is bounded by kFp8MfmaRelErr of their absolute sum per
         // instruction, then FP32 accumulation across the n/128 instructions
         const double nc = ceil(n / 128.0) + 1.0;
-        const double ga = elem == kElemFP8 ? kFp8MfmaRelErr + nc * u / (1.0 - nc * u) : gamma;
+        // (+3u: the epilogue's two multiplies by the scales' mantissas)
+        const double ga =
+            elem == kElemFP8 ? kFp8MfmaRelErr + nc * u / (1.0 - nc * u) + 3.0 * u : gamma;
         const double d = gamma * xm * qn + ga * xl * ql + xl * qd + xe * qn;
         delta[qi] = (float)(d * (1.0 + 1e-6)) + 1e-30f;
     }
@@ -653,8 +659,8 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // (finite inputs).  A key at or above the threshold has a score at or
     // above the threshold's floor, so each 16-query tile first tests the
     // largest score of every lane and is skipped when none is that high.
-    auto epi0 = [&](const v4f (&ac)[MT][NT][E::NACC], const float (&thv)[NT], uint32_t ch,
-                    uint32_t bt) {
+    auto epi0 = [&](const v4f (&ac)[MT][NT][E::NACC], const float (&thv)[NT],
+                    const float (&fqv)[NT], const float (&fr)[MT][4], uint32_t ch, uint32_t bt) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const uint32_t qi = ch * QC + t * 16 + l16;
@@ -663,6 +669,8 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             // when it is at or above the floor (-0 and +0 aside, which only
             // adds a candidate); +inf for the padding queries
             const float thf = thv[t];
+            // FP8: score = (acc * row mantissa) * query mantissa, so the
+            // largest of acc * row mantissa times the query's decides the tile
             float sc[MT][4];
             float mx = -__builtin_inff();
 #pragma unroll
@@ -670,9 +678,17 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     sc[m][v] = E::total(ac[m][t], v);
+                    if constexpr (E::SCALED) sc[m][v] = sc[m][v] * fr[m][v];
                     mx = __builtin_fmaxf(mx, sc[m][v]);
                 }
+            if constexpr (E::SCALED) mx = mx * fqv[t];
             if (__ballot(!(mx < thf)) == 0) continue;
+            if constexpr (E::SCALED) {
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) sc[m][v] = sc[m][v] * fqv[t];
+            }
             // Rare from here: the lane's (row, query) pairs at or above the
             // threshold as bits m*4+v, then appended a round at a time (one
             // pair per lane per round), which keeps this path short in code
@@ -713,7 +729,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // chunk late, so one wave's epilogue overlaps the other's MFMAs.
     const bool late = E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
     v4f accp[MT][NT][E::NACC];
-    float thp[NT];
+    float thp[NT], fqp[NT];
     for (uint32_t bt = blockIdx.x; bt < nblk; bt += gridDim.x) {
         // ---- stationary X: MT x MAXS units per lane (rows past the padded
         // count read the last row; their results are never written) ----
@@ -744,14 +760,27 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     bestq[m][v] = 0xFFFFFFFFu;
                 }
         }
-        // FP8: E8M0 scale of the lane's stationary row in each row tile
+        // FP8: the E8M0 exponent of the scale of the lane's stationary row
+        // in each row tile (applied by the MFMA), and the mantissas of the
+        // scales of the rows the lane's results belong to (4g + v of each
+        // tile; applied by the epilogue)
         uint32_t sa[MT];
+        float fr[MT][4];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             sa[m] = 127u;
             if constexpr (E::SCALED) {
                 const uint32_t row = bt * block_rows + wid * RW + m * 16 + l16;
                 sa[m] = e8m0_of(a.row_scale[row < a.rows_padded ? row : a.rows_padded - 1]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                fr[m][v] = 1.f;
+                if constexpr (E::SCALED) {
+                    const uint32_t row = bt * block_rows + wid * RW + m * 16 + 4 * g + v;
+                    fr[m][v] =
+                        mant_of(a.row_scale[row < a.rows_padded ? row : a.rows_padded - 1]);
+                }
             }
         }
         if constexpr (DMA) {
@@ -781,16 +810,21 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             // registers before the MFMAs: their LDS latency hides under them,
             // and a late epilogue runs after other waves may have begun
             // copying the next chunk into this buffer
-            float thv[NT];
+            float thv[NT], fqv[NT];
             uint32_t sbq[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 thv[t] = 0.f;
+                fqv[t] = 1.f;
                 sbq[t] = 127u;
                 if (DMA && a.mode == 0) thv[t] = thfl(cur)[t * 16 + l16];
-                if constexpr (E::SCALED) sbq[t] = e8m0_of(qsl(cur)[t * 16 + l16]);
+                if constexpr (E::SCALED) {
+                    const float sq = qsl(cur)[t * 16 + l16];
+                    sbq[t] = e8m0_of(sq);
+                    fqv[t] = mant_of(sq);
+                }
             }
-            if (late && ch > ch0) epi0(accp, thp, ch - 1, bt);
+            if (late && ch > ch0) epi0(accp, thp, fqp, fr, ch - 1, bt);
             v4f acc[MT][NT][E::NACC];
 #pragma unroll
             for (int m = 0; m < MT; ++m)
@@ -844,13 +878,14 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
                         thp[t] = thv[t];
+                        fqp[t] = fqv[t];
 #pragma unroll
                         for (int m = 0; m < MT; ++m)
 #pragma unroll
                             for (int j = 0; j < E::NACC; ++j) accp[m][t][j] = acc[m][t][j];
                     }
                 } else {
-                    epi0(acc, thv, ch, bt);
+                    epi0(acc, thv, fqv, fr, ch, bt);
                 }
             }
             // ---- mode 1 and row argmax epilogue: lane holds rows 4*g+v of
@@ -864,7 +899,10 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     const uint32_t row0 = bt * block_rows + wid * RW + m * 16 + 4 * g;
                     float sc[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) sc[v] = E::total(acc[m][t], v);
+                    for (int v = 0; v < 4; ++v) {
+                        sc[v] = E::total(acc[m][t], v);
+                        if constexpr (E::SCALED) sc[v] = (sc[v] * fr[m][v]) * fqv[t];
+                    }
                     if constexpr (AM) {
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
@@ -901,7 +939,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             __syncthreads();
         }
         if constexpr (!AM) {
-            if (late) epi0(accp, thp, ch1 - 1, bt);
+            if (late) epi0(accp, thp, fqp, fr, ch1 - 1, bt);
             if (cq_n) flush();
         }
         if constexpr (AM) {

@@ -17,8 +17,8 @@ struct MfmaScanArgs {
     uint32_t dch;              // 16-byte units per row
     const uint4 *q;            // queries, row-major, q_stride units per query
     uint32_t q_stride;
-    const float *row_scale;    // fp8: per-row power-of-two scale [rows_padded]
-    const float *q_scale;      // fp8: per-query power-of-two scale [nq]
+    const float *row_scale;    // fp8: per-row scale [rows_padded]
+    const float *q_scale;      // fp8: per-query scale [nq]
     uint32_t nq;
     uint32_t mode;             // 0: append candidates, 1: write scores, 2: row argmax
     const unsigned long long *thresh;  // [nq] key thresholds (register staging)
@@ -63,8 +63,8 @@ struct RerankArgs {
 };
 
 void launch_to_bf16(const float4 *x, size_t rows_padded, uint32_t d8, uint4 *y, hipStream_t s);
-// FP8-E4M3 (OCP) copy with a per-row power-of-two scale s_r >= max|x_r| / 448
-// (the least such; an E8M0 block scale of the MFMA): tiled units of
+// FP8-E4M3 (OCP) copy with a per-row scale s_r = max|x_r| / 448 (at least
+// 2^-126; its exponent is the MFMA's E8M0 block scale): tiled units of
 // 16 codes (d16 = ceil(d4 / 4) units per row), scales [rows_padded], and the
 // largest per-row norm of |s_r x~_r - x_r| (f64 bits, atomic max into *err_sq).
 void launch_to_fp8(const float4 *x, size_t rows_padded, uint32_t d4, uint4 *y, float *scale,

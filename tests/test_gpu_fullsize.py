@@ -10,6 +10,10 @@
   base with nlist 4,096 and 20 iterations checked through size-independent
   properties (a permutation, offsets, centroids the normalised means of their
   rows, recall@10 against the exact search at nprobe 8 / 32 / 128).
+- Precision form: 5,000,000 x 768 anisotropic base, 1,024 queries, k=100,
+  over-fetch 400: the exact search against the oracle and FP64 on a sample,
+  BF16 and FP8 scoring + exact rerank at recall@100 = 1 against it, certified
+  BF16 queries identical to it, every returned score the exact FP32 one.
 - Serving: 65,536 x 1,024 (exact batches score on BF16, the k-means
   assignment runs on BF16), mixed exact / IVF requests served alone and all at
   once, equal to each other and to the oracle.
@@ -197,3 +201,49 @@ def test_serving_at_1024_dims_alone_and_mixed(orc):
             assert np.array_equal(gs[r, : len(oi)].view(np.uint32), osim.view(np.uint32))
             checked += 1
     assert checked >= 24
+
+
+def test_c3_precision_form_recall(orc):
+    """5M x 768 anisotropic, 1,024 queries, k = 100, 4k over-fetch: BF16 and
+    FP8 scoring + exact FP32 rerank return the exact search's results
+    (recall@100 = 1 on this workload), certified BF16 queries bit for bit;
+    the exact search equals the oracle on a sample."""
+    _case("c3", rows=5_000_000, dims=768)
+    base = synth.anisotropic(5_000_000, 768, seed=42)
+    qs = synth.anisotropic(1024, 768, seed=43)
+    k, nq = 100, 1024
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, nq, 448)
+    dq = DevBuf.from_array(qs)
+    ei, es = DevBuf(nq * k * 8), DevBuf(nq * k * 4)
+    ds.run(dq, nq, k, ei, es)
+    exact_i, exact_s = ei.to_array((nq, k), np.uint64), es.to_array((nq, k), np.float32)
+    for i in (0, 511):
+        oidx, osim, _ = orc.cpu_search(base, qs[i], k)
+        assert exact_i[i].tolist() == oidx.tolist()
+        assert np.array_equal(exact_s[i].view(np.uint32), osim.view(np.uint32))
+    from nndex_amd import _lib
+    L = _lib.lib()
+    for prec, name in ((1, "bf16"), (2, "fp8")):
+        di, dsim, dfl = DevBuf(nq * k * 8), DevBuf(nq * k * 4), DevBuf(nq)
+        from gpu_util import check
+        check(L.nndex_gpu_index_search_batch_device(b.h, ds.ws, dq.ptr, nq, k, prec, 4 * k,
+                                                    di.ptr, dsim.ptr, dfl.ptr, ds.stream))
+        check(L.nndex_stream_synchronize(ds.stream))
+        gi, gs = di.to_array((nq, k), np.uint64), dsim.to_array((nq, k), np.float32)
+        fl = dfl.to_array((nq,), np.uint8)
+        rq = np.array([len(set(gi[i].tolist()) & set(exact_i[i].tolist())) / k for i in range(nq)])
+        print(f"{name}: recall@{k} {rq.mean():.5f}, certified {1 - fl.mean():.3f}", flush=True)
+        assert rq.min() == 1.0, (name, int((rq < 1).sum()), float(rq.min()))
+        for i in np.nonzero(fl == 0)[0]:
+            assert gi[i].tolist() == exact_i[i].tolist()
+            assert np.array_equal(gs[i].view(np.uint32), exact_s[i].view(np.uint32))
+        # every returned score is its row's exact FP32 score: it equals the
+        # exact search's score of the same row
+        for i in range(0, nq, 97):
+            pos = {int(r): j for j, r in enumerate(exact_i[i].tolist())}
+            for j in range(k):
+                assert gs[i, j] == exact_s[i, pos[int(gi[i, j])]]
+        if name == "bf16":
+            assert (fl == 0).mean() >= 0.9
+    del base
