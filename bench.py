@@ -283,8 +283,6 @@ def ivf_cpu_baseline(m: np.ndarray, queries: np.ndarray, k: int) -> dict:
 def serve_bench(args) -> dict:
     """The serving form (see the module docstring): open-loop Poisson arrivals
     into the continuous-batching loop over a resident index."""
-    import threading
-
     from nndex_amd.serve import EXACT, IVF, Server
 
     L = _lib.lib()
@@ -302,52 +300,55 @@ def serve_bench(args) -> dict:
     log(f"serve: IVF nlist {info.nlist} built in {time.perf_counter() - t0:.2f}s")
     t0 = time.perf_counter()
     srv = Server(h, dims, rows, slots=args.serve_slots, max_batch_queries=2048, max_k=100,
-                 ivf_nprobe_max=32)
+                 ivf_nprobe_max=32, graphs=args.serve_graphs)
     st0 = srv.stats()
     log(f"serve: {st0['slots']} executors of {st0['slot_bytes'] / 2**30:.2f} GiB prepared in "
         f"{time.perf_counter() - t0:.1f}s")
-    rng = np.random.default_rng(2024)
-    n_req = max(1, int(args.serve_rate * args.serve_seconds))
-    gaps = rng.exponential(1.0 / args.serve_rate, size=n_req)
-    sizes = np.clip(np.round(np.exp(rng.uniform(0.0, np.log(2048), size=n_req))), 1,
-                    2048).astype(int)
-    ks = rng.choice([1, 10, 100], size=n_req)
-    modes = rng.integers(0, 2, size=n_req)
-    starts = rng.integers(0, bank.shape[0] - 2048, size=n_req)
     # warm-up: one request of each kind
     for m in (EXACT, IVF):
         srv.wait(srv.submit(bank[:64], 10, m, 32 if m == IVF else 0))
-    srv.reset_stats()
-    tickets = []
-    lock = threading.Lock()
-    t_start = time.perf_counter()
-    due = t_start
-    for i in range(n_req):
-        due += gaps[i]
-        now = time.perf_counter()
-        if due > now:
-            time.sleep(due - now)
-        q = bank[starts[i]:starts[i] + sizes[i]]
-        m = int(modes[i])
-        t = srv.submit(q, int(ks[i]), m, 32 if m == IVF else 0)
-        with lock:
-            tickets.append((t, m, int(sizes[i])))
-        if i % max(1, n_req // 10) == 0:
-            log(f"serve: submitted {i + 1}/{n_req}")
-    offered_s = time.perf_counter() - t_start
-    lat = {EXACT: [], IVF: []}
-    for t, m, _ in tickets:
-        lat[m].append(srv.wait(t)[3])
-    st = srv.stats()
-    srv.close()
-    L.nndex_gpu_index_free(h)
+
+    def run_rate(rate, seconds, seed, progress):
+        """Open-loop Poisson arrivals at `rate` requests/s for `seconds`;
+        every request waited for; the loop's statistics over the run."""
+        rng = np.random.default_rng(seed)
+        n_req = max(1, int(rate * seconds))
+        gaps = rng.exponential(1.0 / rate, size=n_req)
+        sizes = np.clip(np.round(np.exp(rng.uniform(0.0, np.log(2048), size=n_req))), 1,
+                        2048).astype(int)
+        ks = rng.choice([1, 10, 100], size=n_req)
+        modes = rng.integers(0, 2, size=n_req)
+        starts = rng.integers(0, bank.shape[0] - 2048, size=n_req)
+        srv.reset_stats()
+        tickets = []
+        t_start = time.perf_counter()
+        due = t_start
+        for i in range(n_req):
+            due += gaps[i]
+            now = time.perf_counter()
+            if due > now:
+                time.sleep(due - now)
+            m = int(modes[i])
+            t = srv.submit(bank[starts[i]:starts[i] + sizes[i]], int(ks[i]), m,
+                           32 if m == IVF else 0)
+            tickets.append((t, m))
+            if progress and i % max(1, n_req // 10) == 0:
+                log(f"serve: submitted {i + 1}/{n_req}")
+        offered_s = time.perf_counter() - t_start
+        lat = {EXACT: [], IVF: []}
+        for t, m in tickets:
+            lat[m].append(srv.wait(t)[3])
+        st = srv.stats()
+        return st, lat, float(sizes.sum()) / offered_s
+
+    st, lat, offered = run_rate(args.serve_rate, args.serve_seconds, 2024, True)
     res = {"workload": f"{rows}x{dims} Gaussian mixture (1000 centres, spread 0.5, "
                        "unit-normalised), FP32 + BF16 copies, IVF nlist "
                        f"{int(info.nlist)}; Poisson arrivals at {args.serve_rate:g}/s for "
                        f"{args.serve_seconds:g} s; rows per request log-uniform 1..2048, "
                        "k in {1,10,100}, half exact / half IVF nprobe 32",
            "requests": int(st["requests"]), "queries": int(st["queries"]),
-           "offered_rows_per_s": round(float(sizes.sum()) / offered_s, 1),
+           "offered_rows_per_s": round(offered, 1),
            "queries_per_s": round(st["queries_per_s"], 1),
            "requests_per_s": round(st["requests_per_s"], 2),
            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
@@ -359,7 +360,29 @@ def serve_bench(args) -> dict:
            "batches": int(st["batches"]),
            "mean_batch_queries": round(st["mean_batch_queries"], 1),
            "executors": int(st["slots"]),
-           "executor_bytes": int(st["slot_bytes"])}
+           "executor_bytes": int(st["slot_bytes"]),
+           "graph_batches": int(st.get("graph_batches", 0))}
+    if args.serve_sweep:
+        # capacity: higher arrival rates for shorter runs; a rate is
+        # sustained when the loop serves at least 97% of the rows offered
+        # (its queue does not grow)
+        sweep = []
+        for rate in (300.0, 450.0, 600.0):
+            st2, _, off2 = run_rate(rate, args.serve_sweep_seconds, int(rate), False)
+            row = {"arrivals_per_s": rate, "offered_rows_per_s": round(off2, 1),
+                   "served_rows_per_s": round(st2["queries_per_s"], 1),
+                   "p50_ms": round(st2["p50_ms"], 2), "p99_ms": round(st2["p99_ms"], 2),
+                   "mean_batch_queries": round(st2["mean_batch_queries"], 1),
+                   "sustained": bool(st2["queries_per_s"] >= 0.97 * off2)}
+            log(f"serve sweep: {row}")
+            sweep.append(row)
+        ok = [r["served_rows_per_s"] for r in sweep if r["sustained"]]
+        ok.append(res["queries_per_s"] if res["queries_per_s"] >= 0.97 * offered else 0.0)
+        res["capacity_sweep"] = sweep
+        res["max_sustained_rows_per_s"] = max(ok)
+        res["saturated_rows_per_s"] = max(r["served_rows_per_s"] for r in sweep)
+    srv.close()
+    L.nndex_gpu_index_free(h)
     log(f"serve: {res}")
     return res
 
@@ -524,6 +547,11 @@ def main():
                     help="request arrivals per second (Poisson)")
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--no-serve-sweep", dest="serve_sweep", action="store_false",
+                    help="skip the serving capacity sweep (arrivals 300/450/600 per s)")
+    ap.add_argument("--serve-sweep-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-graphs", action="store_true",
+                    help="serve exact batches by replaying captured HIP graphs")
     ap.add_argument("--profile-batch", choices=["f32", "bf16", "fp8"], default=None,
                     help="only run the batched search at this scoring precision (for profilers)")
     ap.add_argument("--profile-workload", choices=["1m_x_384", "5m_x_768"], default="1m_x_384",

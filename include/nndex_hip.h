@@ -319,9 +319,10 @@ typedef struct nndex_server_options {
     uint32_t max_k;             /* largest k served; 0 = 100 */
     uint32_t ivf_nprobe_max;    /* 0 = exact requests only; else IVF up to this nprobe */
     uint64_t hbm_budget_bytes;  /* device bytes for all executors; 0 = 16 GiB */
-    uint32_t use_graphs;        /* 1 = exact batches above 8 rows replay HIP
-                                   graphs captured per (rows, k) power-of-two
-                                   bucket; 0 = eager launches */
+    uint32_t use_graphs;        /* 1 = exact batches above 4 rows replay HIP
+                                   graphs captured per (rows, k) bucket: rows
+                                   to a multiple of 16 (<= 128) or 64, k to a
+                                   power of two; 0 = eager launches */
 } nndex_server_options;
 
 typedef struct nndex_server_stats {

@@ -15,9 +15,9 @@
 //     total, so the first k of a top-k' list are the top-k).
 //   * Work that arrives while batches run is admitted as soon as an executor
 //     frees, so batches of different kinds overlap on the device.
-//   * With use_graphs, exact batches above 8 rows replay a HIP graph captured
-//     per executor for
-//     (rows rounded up to a power of two, k rounded up to a power of two):
+//   * With use_graphs, exact batches above 4 rows replay a HIP graph captured
+//     per executor for (rows rounded up to a multiple of 16 up to 128 rows and
+//     of 64 above, k rounded up to a power of two):
 //     padding rows repeat the last row, each row keeps its own first k
 //     results, and rows the certificate leaves open run the exact fused scan.
 //   * Admission by HBM footprint: an executor's device memory is fixed at
@@ -306,10 +306,11 @@ bool Server::execute(Executor &ex, std::vector<Segment> &batch) {
     Precision prec;
     if (mode == 0 && opt_.use_graphs && nq > kFusedBatchMax &&
         index_->exact_batch_precision(prec)) {
-        // replay a captured exact batch: rows padded to a power of two (with
-        // copies of the last row), k to a power of two (results are prefixes)
-        uint32_t qb = 16, kb = 1;
-        while (qb < nq) qb <<= 1;
+        // replay a captured exact batch: rows padded (with copies of the
+        // last row) to a multiple of 16 up to 128 rows and of 64 above, so
+        // at most one query chunk of the scan is padding; k to a power of
+        // two (results are prefixes)
+        uint32_t qb = nq <= 128 ? (nq + 15) / 16 * 16 : (nq + 63) / 64 * 64, kb = 1;
         if (qb > opt_.max_batch_queries) qb = opt_.max_batch_queries;
         while (kb < k) kb <<= 1;
         if (kb > opt_.max_k) kb = opt_.max_k;

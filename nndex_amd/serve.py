@@ -27,7 +27,8 @@ class Server:
 
     Queries are normalised, zero-padded rows (``padded_dims`` columns), as at
     the backend boundary.  ``graphs=True`` replays captured HIP graphs for exact
-    batches above 8 rows (rows and k rounded up to powers of two).  ``submit`` returns a ticket at once; ``wait``
+    batches above 4 rows (rows rounded up to a multiple of 16 up to 128 and of
+    64 above, k to a power of two).  ``submit`` returns a ticket at once; ``wait``
     returns ``(indices[uint64, rows x k], similarities[float32, rows x k],
     counts[uint64, rows], latency_ms)``.
     """
