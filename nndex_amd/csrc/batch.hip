@@ -546,6 +546,14 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
         ch0 = blockIdx.y * nch_per;
         ch1 = ch0 + nch_per < nch_all ? ch0 + nch_per : nch_all;
     }
+    // Work items: the row tiles of the full rounds (every block one tile per
+    // round) take all chunks; the tiles of a last, partial round are split
+    // into tail_split pieces of the chunks, so the blocks finish together
+    // (at 1M rows, 3,907 tiles over 256 blocks leave 67 for a 16th round)
+    const uint32_t tsplit = (!SPLIT && DMA && a.tail_split > 1) ? a.tail_split : 1u;
+    const uint32_t nfull = tsplit > 1 ? nblk / gridDim.x * gridDim.x : nblk;
+    const uint32_t nwork = nfull + (nblk - nfull) * tsplit;
+    const uint32_t tail_per = (nch_all + tsplit - 1) / tsplit;
     constexpr uint32_t RU = 4 * MAXS;    // staged units per query (>= dch)
     constexpr uint32_t chunk_units = QC * RU;
     if (a.nq == 0 || dch == 0 || ch0 >= ch1) return;
@@ -730,7 +738,15 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     const bool late = E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
     v4f accp[MT][NT][E::NACC];
     float thp[NT], fqp[NT];
-    for (uint32_t bt = blockIdx.x; bt < nblk; bt += gridDim.x) {
+    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        uint32_t bt = w;
+        if (w >= nfull) {  // block-uniform
+            const uint32_t r = w - nfull, piece = r % tsplit;
+            bt = nfull + r / tsplit;
+            ch0 = piece * tail_per;
+            ch1 = ch0 + tail_per < nch_all ? ch0 + tail_per : nch_all;
+            if (ch0 >= ch1) continue;
+        }
         // ---- stationary X: MT x MAXS units per lane (rows past the padded
         // count read the last row; their results are never written) ----
         uint4 xr[MT][MAXS];

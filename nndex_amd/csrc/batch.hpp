@@ -38,6 +38,10 @@ struct MfmaScanArgs {
     // query splits (grid y): each block row scans its share of the query
     // chunks (0 or 1: one share)
     uint32_t qsplit;
+    // modes 0 and 1 without qsplit: the row tiles of a last, partial round
+    // of the grid are split into this many pieces of the query chunks
+    // (0 or 1: not split)
+    uint32_t tail_split;
     // modes 0 and 1: the batch's chunk image (launch_chunk_image for this
     // element type, width and variant), which the scan copies into LDS;
     // `thresh_f` (mode 0) and `q_scale` (FP8) are then read a whole chunk at

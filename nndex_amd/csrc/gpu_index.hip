@@ -441,11 +441,27 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         // one 8-wave block per CU (register-bound); a scan with fewer row
         // tiles than CUs also splits the queries over blocks (grid y),
         // choosing the split that finishes in the fewest block waves per
-        // unit of work
+        // unit of work; with more tiles, the tiles of the last partial round
+        // are split the same way (tail_split), so no block idles a round
+        const uint32_t nch = static_cast<uint32_t>((n + qc - 1) / qc);
         auto grid_for = [&](uint32_t rp) {
             const uint32_t blocks = (rp + br - 1) / br;
             const uint32_t cap = ncu_;
             a.qsplit = 1;
+            a.tail_split = 1;
+            if (blocks > cap && blocks % cap) {
+                // rounds of tile work, each piece also paying its own loads
+                // of the stationary rows (about 2% of a tile)
+                const uint32_t full = blocks / cap, rem = blocks % cap;
+                double best = full + 1.0;
+                for (uint32_t q = 2; q <= 8 && q <= nch; ++q) {
+                    const double t = full + (double)((rem * q + cap - 1) / cap) / q + 0.02 * q;
+                    if (t < best - 1e-9) {
+                        best = t;
+                        a.tail_split = q;
+                    }
+                }
+            }
             if (blocks < cap) {
                 const uint32_t max_split = static_cast<uint32_t>((n + 15) / 16);
                 double best = 1.0;

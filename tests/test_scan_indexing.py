@@ -290,3 +290,62 @@ def test_chunk_image_inverts_the_lds_swizzle():
                     n = t * 16 + l16
                     read_slot = t * 16 + ((l16 ^ (4 * (s_ & 3) + g)) & 15)
                     assert read_slot == (n & ~15) | ((n ^ u) & 15)
+
+
+def tail_split_choice(blocks, cap, nch):
+    """gpu_index.hip grid_for: the split of the last partial round's tiles."""
+    best, split = None, 1
+    if blocks > cap and blocks % cap:
+        full, rem = blocks // cap, blocks % cap
+        best = full + 1.0
+        for q in range(2, min(8, nch) + 1):
+            t = full + -(-rem * q // cap) / q + 0.02 * q
+            if t < best - 1e-9:
+                best, split = t, q
+    return split
+
+
+@pytest.mark.parametrize("rows,block_rows,nq,qc,cap", [
+    (1_000_000, 256, 2048, 32, 256),   # 3,907 tiles: 67 in the last round
+    (5_000_000, 128, 1024, 32, 256),
+    (5_000_000, 256, 1024, 32, 256),
+    (70_000, 128, 40, 16, 256),        # fewer tiles than blocks: no tail split
+    (300_000, 256, 17, 16, 256),       # one chunk: nothing to split
+    (2_000_000, 128, 333, 16, 256),
+])
+def test_mfma_work_items_cover_every_tile_and_chunk_once(rows, block_rows, nq, qc, cap):
+    """mfma_scan_kernel's work items: the full rounds' tiles take every chunk,
+    the last round's tiles are split into tail_split chunk ranges; every
+    (row tile, chunk) pair is scanned exactly once and no block runs more
+    than the rounds the host planned."""
+    rows_padded = -(-rows // 64) * 64
+    nblk = -(-rows_padded // block_rows)
+    nch = -(-nq // qc)
+    grid = min(nblk, cap)
+    ts = tail_split_choice(nblk, cap, nch)
+    nfull = nblk // grid * grid if ts > 1 else nblk
+    nwork = nfull + (nblk - nfull) * ts
+    per = -(-nch // ts)
+    seen = np.zeros((nblk, nch), np.int32)
+    busy = np.zeros(grid)
+    for b in range(grid):
+        for w in range(b, nwork, grid):
+            if w < nfull:
+                bt, c0, c1 = w, 0, nch
+            else:
+                r = w - nfull
+                bt, c0 = nfull + r // ts, (r % ts) * per
+                c1 = min(c0 + per, nch)
+                if c0 >= c1:
+                    continue
+            seen[bt, c0:c1] += 1
+            busy[b] += (c1 - c0) / nch
+    assert np.all(seen == 1)
+    # the busiest block does no more tile-rounds than the plan
+    # (pieces are whole chunks: ceil(nch / tail_split) of nch each)
+    plan = nblk // grid + (-(-(nblk % grid) * ts // grid) * per / nch if nblk % grid else 0)
+    assert busy.max() <= plan + 1e-9
+    if ts > 1:
+        assert busy.max() < nblk // grid + 1   # better than a whole extra round
+    if (rows, nq) == (1_000_000, 2048):
+        assert ts == 3
