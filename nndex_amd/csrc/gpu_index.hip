@@ -1262,8 +1262,17 @@ const IvfIndex &GpuIndex::build_ivf() {
         NNDEX_HIP_CHECK(hipMemcpy(&ebits, xerr_sq_ + 1, sizeof(ebits), hipMemcpyDeviceToHost));
         x_err = std::sqrt(__builtin_bit_cast(double, ebits));
     }
+    Fp8Rows fp8;
+    {  // FP8 copy: the k-means++ pre-test (half the bytes of the BF16 one)
+        ensure_fp8(stream_);
+        unsigned long long e8 = 0;
+        NNDEX_HIP_CHECK(hipMemcpy(&e8, xerr8_sq_, sizeof(e8), hipMemcpyDeviceToHost));
+        fp8.codes = xfp8_;
+        fp8.scale = xscale_;
+        fp8.err = std::sqrt(__builtin_bit_cast(double, e8));
+    }
     ivf_build(*ivf, x_, xbf16_, static_cast<uint32_t>(rows_), static_cast<uint32_t>(pdims_),
-              static_cast<uint32_t>(nlist), static_cast<uint32_t>(iters), x_max, x_err, ncu_,
+              static_cast<uint32_t>(nlist), static_cast<uint32_t>(iters), x_max, x_err, fp8, ncu_,
               stream_);
     ivf->cindex = from_device_rows(ivf->centroids, nlist, pdims_, device_);
     ivf_ = std::move(ivf);

@@ -246,6 +246,61 @@ __global__ __launch_bounds__(kKppRows) void kpp_update_kernel(const float4 *__re
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// The same pass with the pre-test on the FP8 copy (half the bytes of the
+// BF16 one): a = s_r * sum(code_i * c_i) over eight FMA chains (codes decoded
+// by v_cvt_pk_f32_fp8, OCP E4M3), against the FP32 centroid; |a - x.c| <=
+// gamma |x~||c| + |x~ - x||c|, folded with the reference value's own error
+// into `margin` by the caller.
+__global__ __launch_bounds__(kKppRows) void kpp_update_fp8_kernel(
+    const float4 *__restrict__ x, const uint4 *__restrict__ x8, const float *__restrict__ xs,
+    uint32_t rows, uint32_t d4, const float4 *__restrict__ last, double margin,
+    float *__restrict__ min_dist, double *__restrict__ partial) {
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    __shared__ double red[kKppRows / 64];
+    const uint32_t row = blockIdx.x * kKppRows + threadIdx.x;
+    double md = 0.0;
+    if (row < rows) {
+        float m = min_dist[row];
+        bool skip = false;
+        if (m < 3.0e38f) {
+            const uint32_t d16 = (d4 + 3) / 4;
+            const uint4 *b = x8 + (size_t)(row / 64) * d16 * 64 + row % 64;
+            const_as<f32x4> *cb = as_constant(reinterpret_cast<const f32x4 *>(last));
+            float l[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (uint32_t u = 0; u < d16; ++u) {
+                const uint4 w = b[(size_t)u * 64];
+                const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t c4 = 4 * u + j;  // the float4 these four codes cover
+                    if (c4 >= d4) break;            // (wave-uniform) padding codes
+                    const f32x4 q = cb[c4];
+                    const v2f lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)ws[j], false);
+                    const v2f hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)ws[j], true);
+                    l[2 * j] = __builtin_fmaf(lo.x, q.x, l[2 * j]);
+                    l[2 * j] = __builtin_fmaf(lo.y, q.y, l[2 * j]);
+                    l[2 * j + 1] = __builtin_fmaf(hi.x, q.z, l[2 * j + 1]);
+                    l[2 * j + 1] = __builtin_fmaf(hi.y, q.w, l[2 * j + 1]);
+                }
+            }
+            const float a =
+                xs[row] * (((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + (l[6] + l[7])));
+            skip = (1.0 - (double)a) - margin > (double)m * (1.0 + 1e-12) + 1e-30;
+        }
+        if (!skip) {
+            float d = 1.0f - tiled_dot(x, row, d4, last);
+            if (d < 0.0f) d = 0.0f;
+            if (d < m) {
+                m = d;
+                min_dist[row] = m;
+            }
+        }
+        md = (double)m;
+    }
+    const double t = kpp_block_sum<kKppRows>(md, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
 // One block of kKppBlock threads: total, generator step, then the first row
 // whose cumulative distance reaches the target, found top-down (ranges of
 // update blocks, update blocks, rows) with block-wide prefix sums.  Writes
@@ -863,7 +918,7 @@ static uint32_t ivf_cw(uint32_t k) {
 
 void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows,
                uint32_t pdims, uint32_t nlist, uint32_t iters, double x_max, double x_err,
-               uint32_t ncu, hipStream_t s) {
+               const Fp8Rows &fp8, uint32_t ncu, hipStream_t s) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t d4 = pdims / 4, d8 = pdims / 8;
     // assignment: FP32 MFMA up to 512 padded dims, BF16 MFMA up to 1024 (both
@@ -953,16 +1008,29 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                                    k4 * u / (1.0 - k4 * u) * x_max * x_max +
                                    1.01 * u * (1.0 + x_max * x_max)) *
                                   (1.0 + 1e-6);
-        const uint4 *kpp_xb = (xbf16 && d4 % 2 == 0) ? xbf16 : nullptr;
+        const uint4 *kpp_xb = (xbf16 && d4 % 2 == 0 && !fp8.codes) ? xbf16 : nullptr;
+        // the FP8 pre-test: gamma over eight chains of 2 d16 terms, the
+        // summation tree and the scale multiply, plus |x~ - x||c| (c is a
+        // row: |c| <= x_max), plus the reference value's error as above
+        const double k16 = 2.0 * ((d4 + 3) / 4) + 4.0;
+        const double kpp_margin8 =
+            (k16 * u / (1.0 - k16 * u) * (x_max + fp8.err) * x_max + fp8.err * x_max +
+             k4 * u / (1.0 - k4 * u) * x_max * x_max + 1.01 * u * (1.0 + x_max * x_max)) *
+            (1.0 + 1e-6);
         // the choice of centroid c also writes its BF16 rounding (lastb) for
         // pass c + 1
         const uint32_t kb = (rows + kKppRows - 1) / kKppRows;
         if (kpp_xb)
             hipLaunchKernelGGL(bf16_round_row_kernel, dim3(1), dim3(64), 0, s, cent, d4, lastb);
         for (uint32_t c = 1; c < nlist; ++c) {
-            hipLaunchKernelGGL(kpp_update_kernel, dim3(kb), dim3(kKppRows), 0, s, x, kpp_xb, rows,
-                               d4, cent + (size_t)(c - 1) * d4, lastb, kpp_margin, min_dist,
-                               partial);
+            if (fp8.codes)
+                hipLaunchKernelGGL(kpp_update_fp8_kernel, dim3(kb), dim3(kKppRows), 0, s, x,
+                                   fp8.codes, fp8.scale, rows, d4, cent + (size_t)(c - 1) * d4,
+                                   kpp_margin8, min_dist, partial);
+            else
+                hipLaunchKernelGGL(kpp_update_kernel, dim3(kb), dim3(kKppRows), 0, s, x, kpp_xb,
+                                   rows, d4, cent + (size_t)(c - 1) * d4, lastb, kpp_margin,
+                                   min_dist, partial);
             hipLaunchKernelGGL(kpp_choose_kernel, dim3(1), dim3(kKppBlock), 0, s, x, rows, d4,
                                min_dist, partial, kb, rng, cent, c, kpp_xb ? lastb : nullptr);
         }

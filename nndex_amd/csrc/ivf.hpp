@@ -106,12 +106,21 @@ size_t ivf_scan_scratch_bytes(size_t nq, size_t nprobe, size_t nlist, size_t row
 // rocPRIM temporary bytes for sorting `pairs` (list, pair) keys of `nlist` lists.
 size_t ivf_sort_bytes(size_t pairs, size_t nlist);
 
+// FP8 copy of the rows for the k-means++ pre-test: tiled units of 16 E4M3
+// codes, per-row scales and the largest row error norm |s_r x~_r - x_r|.
+struct Fp8Rows {
+    const uint4 *codes = nullptr;
+    const float *scale = nullptr;
+    double err = 0.0;
+};
 // Build the index of `rows` tiled rows (pdims padded columns) on stream s.
 // x_max is the largest row norm; above 512 padded dims the assignment runs on
-// the BF16 copy xbf16, whose largest row rounding-error norm is x_err.
+// the BF16 copy xbf16, whose largest row rounding-error norm is x_err; the
+// k-means++ passes test rows on the FP8 copy when fp8.codes is set, else on
+// the BF16 copy.
 void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows,
                uint32_t pdims, uint32_t nlist, uint32_t iters, double x_max, double x_err,
-               uint32_t ncu, hipStream_t s);
+               const Fp8Rows &fp8, uint32_t ncu, hipStream_t s);
 // Install a saved build (host arrays, already validated: offsets[0] = 0,
 // non-decreasing, offsets[nlist] = rows; orig a permutation; assign consistent
 // with both) and gather the reordered copy from the resident rows x.
