@@ -472,31 +472,28 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // X-stationary MFMA scan.  NW waves per block, RW rows per wave (multiple of
 // 16), MAXS 16-byte units per lane per row (units per row <= 4 * MAXS; lane
 // group g = lane/16 owns units 4s+g), NT 16-query MFMA tiles per query chunk.
-// Mode 0 appends candidates, mode 1 writes scores: their query chunks come
-// from the batch's chunk image (chunk_image_kernel) by LDS copies issued
-// NB - 1 chunks ahead into a ring of NB buffers, so a chunk's copies have
-// NB - 2 chunks of MFMA work to land in and no registers hold them.  The
-// AM variant (mode 2) keeps each row's two best queries and writes the row
-// argmax; it reads row-major queries (centroids) and stages the next chunk
-// through registers into the other of two buffers.
+// Mode 0 appends candidates, mode 1 writes scores; the AM variant (mode 2)
+// keeps each row's two best queries (centroids) and writes the row argmax.
+// Query chunks come from the batch's chunk image (chunk_image_kernel) by LDS
+// copies issued NB - 1 chunks ahead into a ring of NB buffers, so a chunk's
+// copies have NB - 2 chunks of MFMA work to land in and no registers hold
+// them.
 // ---------------------------------------------------------------------------
 // Entries of a wave's LDS candidate queue (12 bytes each).
 constexpr uint32_t kCandQueue = 128;
 
 // LDS of one query buffer in 16-byte units: [4*steps unit rows][QC slots] of
-// query units, then the chunk's thresholds (candidate mode: QC/4 units of
-// float floors; the row argmax's register staging keeps QC/2 units of
-// 64-bit keys) and QC/4 units of query scales for the scaled (FP8) form.
+// query units, then QC/4 units of the chunk's float threshold floors
+// (candidate mode) and QC/4 units of query scales for the scaled (FP8) form.
 template <typename E, int MAXS, int NT, bool AM>
 constexpr uint32_t mfma_buf_units() {
-    return (uint32_t)(MAXS * 4 * 16 * NT + (AM ? 8 * NT : 4 * NT) + (E::SCALED ? 4 * NT : 0));
+    return (uint32_t)(MAXS * 4 * 16 * NT + 4 * NT + (E::SCALED ? 4 * NT : 0));
 }
 // Buffers in the ring: as many as fit 160 KiB beside the candidate queues
-// (at most 4); the row argmax keeps its two.
+// (at most 4; the row argmax has no queues).
 template <typename E, int NW, int MAXS, int NT, bool AM>
 constexpr uint32_t mfma_nbuf() {
-    if (AM) return 2;
-    constexpr size_t queue = (size_t)NW * kCandQueue * 12;
+    constexpr size_t queue = AM ? 0 : (size_t)NW * kCandQueue * 12;
     constexpr size_t b = (size_t)mfma_buf_units<E, MAXS, NT, AM>() * 16, lds = 160 * 1024;
     return 4 * b + queue <= lds ? 4 : (3 * b + queue <= lds ? 3 : 2);
 }
@@ -510,7 +507,6 @@ constexpr size_t mfma_lds_bytes() {
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
 __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     static_assert(!(AM && E::SCALED), "the row argmax takes unscaled scores");
-    constexpr bool DMA = !AM;            // LDS copies from the chunk image
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
     constexpr uint32_t QC = 16 * NT;     // queries per chunk
     // LDS slots per unit row: unpadded, so the 16 lanes of a group read one
@@ -519,10 +515,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // spreads the transposed commit writes over the banks
     constexpr uint32_t kStride = QC;
     constexpr uint32_t NTH = NW * 64;
-    constexpr int STG = (int)((QC * 4 * MAXS + NTH - 1) / NTH);  // staged units per thread
-    // threshold units per buffer: four float floors each (LDS copies), or
-    // two 64-bit keys each (register staging)
-    constexpr uint32_t TU = DMA ? QC / 4 : QC / 2;
+    constexpr uint32_t TU = QC / 4;      // threshold units (four float floors each) per buffer
     constexpr uint32_t SU = E::SCALED ? QC / 4 : 0;  // query-scale units per buffer
     constexpr uint32_t NB = mfma_nbuf<E, NW, MAXS, NT, AM>();
     extern __shared__ uint4 qlds[];      // NB buffers x ([4*steps units][kStride] + TU + SU)
@@ -550,16 +543,11 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // round) take all chunks; the tiles of a last, partial round are split
     // into tail_split pieces of the chunks, so the blocks finish together
     // (at 1M rows, 3,907 tiles over 256 blocks leave 67 for a 16th round)
-    const uint32_t tsplit = (!SPLIT && DMA && a.tail_split > 1) ? a.tail_split : 1u;
+    const uint32_t tsplit = (!SPLIT && !AM && a.tail_split > 1) ? a.tail_split : 1u;
     const uint32_t nfull = tsplit > 1 ? nblk / gridDim.x * gridDim.x : nblk;
     const uint32_t nwork = nfull + (nblk - nfull) * tsplit;
     const uint32_t tail_per = (nch_all + tsplit - 1) / tsplit;
-    constexpr uint32_t RU = 4 * MAXS;    // staged units per query (>= dch)
-    constexpr uint32_t chunk_units = QC * RU;
     if (a.nq == 0 || dch == 0 || ch0 >= ch1) return;
-    auto thl = [&](uint32_t buf) {
-        return reinterpret_cast<unsigned long long *>(qlds + buf * buf_units + q_units);
-    };
     auto thfl = [&](uint32_t buf) {
         return reinterpret_cast<const float *>(qlds + buf * buf_units + q_units);
     };
@@ -586,14 +574,6 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
         cq_n = 0;
     };
 
-    // register staging (row argmax): zero the padding units of both buffers
-    // once (units >= dch); the chunk image carries its own zeros
-    if constexpr (!DMA) {
-        for (uint32_t i = tid; i < 2 * buf_units; i += NTH) {
-            const uint32_t r = i % buf_units;
-            if (r < q_units && r / kStride >= dch) qlds[i] = make_uint4(0, 0, 0, 0);
-        }
-    }
     // LDS copies of one chunk from the image: NI copies of 1 KiB (64 units)
     // for the queries, one of QC float thresholds (mode 0) and one of QC
     // scales (scaled form), copy i issued by wave i % NW; my_cnt = this
@@ -618,49 +598,6 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                       dst + (q_units + TU) * 16);
         }
     };
-
-    // Staging of the next chunk (queries and their thresholds).  The loads
-    // are unconditional (clamped addresses; out-of-range values are dropped
-    // at the commit), so the count of loads in flight is the same on every
-    // path and the first MFMA of a chunk does not wait for them.
-    const unsigned long long *thsrc =
-        a.mode == 0 ? a.thresh : reinterpret_cast<const unsigned long long *>(a.q);
-    uint4 stg[STG];
-    unsigned long long stg_th;
-    // Staging index i covers query n = i / RU, unit c = i % RU, with RU =
-    // 4 * MAXS a compile-time count (units c >= dch are zero), so the split
-    // is a multiply and shift rather than a division by the run-time width.
-    auto fetch = [&](uint32_t ch) {
-#pragma unroll
-        for (int j = 0; j < STG; ++j) {
-            const uint32_t i = tid + j * NTH;
-            const uint32_t ic = i < chunk_units ? i : chunk_units - 1;
-            const uint32_t n = ic / RU, c0 = ic - n * RU;
-            const uint32_t c = c0 < dch ? c0 : dch - 1;
-            uint32_t qi = ch * QC + n;
-            qi = qi < a.nq ? qi : a.nq - 1;
-            stg[j] = a.q[(size_t)qi * a.q_stride + c];
-        }
-        uint32_t qt = ch * QC + tid % QC;
-        qt = qt < a.nq ? qt : a.nq - 1;
-        stg_th = thsrc[qt];
-    };
-    auto commit = [&](uint32_t buf, uint32_t ch) {
-        uint4 *dst = qlds + buf * buf_units;
-#pragma unroll
-        for (int j = 0; j < STG; ++j) {
-            const uint32_t i = tid + j * NTH;
-            if (i < chunk_units) {
-                const uint32_t n = i / RU, c = i - n * RU;
-                if (c < dch)
-                    dst[c * kStride + ((n & ~15u) | ((n ^ c) & 15u))] =
-                        ch * QC + n < a.nq ? stg[j] : make_uint4(0, 0, 0, 0);
-            }
-        }
-        if (tid < QC)
-            thl(buf)[tid] = (a.mode == 0 && ch * QC + tid < a.nq) ? stg_th : ~0ull;
-    };
-
 
     // Mode 0 epilogue of one chunk: append the (row, query) pairs whose keys
     // reach the query's threshold to the wave's queue.  Scores are finite
@@ -799,28 +736,19 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 }
             }
         }
-        if constexpr (DMA) {
-            // the stationary rows (and scales) have landed: a wait the
-            // compiler sees, so it adds none for them inside the chunk loop
-            // (there it would also wait for the copies in flight)
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
-            // the previous tile left no copies in flight (its last chunk
-            // waited for all) and every wave is done reading its buffers
-            __syncthreads();
-            for (uint32_t j = 0; j + 1 < NB && ch0 + j < ch1; ++j) dma_chunk(ch0 + j, j);
-            wait_vm<0>();
-            __syncthreads();
-        } else {
-            __syncthreads();  // previous block tile done with both buffers
-            fetch(ch0);
-            commit(0, ch0);
-            __syncthreads();
-        }
+        // the stationary rows (and scales) have landed: a wait the compiler
+        // sees, so it adds none for them inside the chunk loop (there it
+        // would also wait for the copies in flight)
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        // the previous tile left no copies in flight (its last chunk waited
+        // for all) and every wave is done reading its buffers
+        __syncthreads();
+        for (uint32_t j = 0; j + 1 < NB && ch0 + j < ch1; ++j) dma_chunk(ch0 + j, j);
+        wait_vm<0>();
+        __syncthreads();
         for (uint32_t ch = ch0; ch < ch1; ++ch) {
-            const bool more = ch + 1 < ch1;
             const uint32_t jj = ch - ch0;
-            const uint32_t cur = DMA ? jj % NB : (jj & 1);
-            if constexpr (!DMA) fetch(more ? ch + 1 : ch);
+            const uint32_t cur = jj % NB;
             const uint4 *src = qlds + cur * buf_units;
             // the chunk's thresholds (mode 0) and FP8 query scales into
             // registers before the MFMAs: their LDS latency hides under them,
@@ -833,7 +761,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 thv[t] = 0.f;
                 fqv[t] = 1.f;
                 sbq[t] = 127u;
-                if (DMA && a.mode == 0) thv[t] = thfl(cur)[t * 16 + l16];
+                if (!AM && a.mode == 0) thv[t] = thfl(cur)[t * 16 + l16];
                 if constexpr (E::SCALED) {
                     const float sq = qsl(cur)[t * 16 + l16];
                     sbq[t] = e8m0_of(sq);
@@ -940,18 +868,14 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 }
             }
-            if constexpr (DMA) {
-                // chunk ch + NB - 1 into the buffer chunk ch - 1 used (every
-                // wave passed the barrier after reading it); then the copies
-                // of chunk ch + 1 must have landed before the barrier, while
-                // those of the chunks after it (ahead - 1 of them, each
-                // my_cnt copies, issued later in order) may stay in flight
-                if (ch + NB - 1 < ch1) dma_chunk(ch + NB - 1, (jj + NB - 1) % NB);
-                const uint32_t ahead = ch + NB - 1 < ch1 ? NB - 1 : ch1 - 1 - ch;
-                wait_vm_dyn(ahead >= 2 ? (ahead - 1) * my_cnt : 0);
-            } else {
-                if (more) commit(cur ^ 1, ch + 1);
-            }
+            // chunk ch + NB - 1 into the buffer chunk ch - 1 used (every wave
+            // passed the barrier after reading it); then the copies of chunk
+            // ch + 1 must have landed before the barrier, while those of the
+            // chunks after it (ahead - 1 of them, each my_cnt copies, issued
+            // later in order) may stay in flight
+            if (ch + NB - 1 < ch1) dma_chunk(ch + NB - 1, (jj + NB - 1) % NB);
+            const uint32_t ahead = ch + NB - 1 < ch1 ? NB - 1 : ch1 - 1 - ch;
+            wait_vm_dyn(ahead >= 2 ? (ahead - 1) * my_cnt : 0);
             __syncthreads();
         }
         if constexpr (!AM) {
@@ -1447,9 +1371,7 @@ static void launch_mfma_split(const MfmaScanArgs &a, uint32_t grid, hipStream_t 
     // the ring of query buffers, then the waves' candidate queues
     constexpr size_t lds = mfma_lds_bytes<E, NW, MAXS, NT, AM>();
     static_assert(lds <= 160 * 1024, "LDS of the MFMA scan");
-    if constexpr (!AM) {
-        if (!a.qimg) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the MFMA scan needs a chunk image");
-    }
+    if (!a.qimg) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the MFMA scan needs a chunk image");
     auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM, SPLIT>;
     // once per instantiation (also keeps the call out of graph captures);
     // serving executors launch from several threads, so the first call sets
@@ -1546,7 +1468,8 @@ static void launch_mfma_case(const MfmaScanArgs &a, int elem, uint32_t grid, hip
     return launch_mfma_t<F32E, 8, 16, ST, t.f32_nt>(a, grid, s);
 }
 
-void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_t &ru) {
+void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_t &ru,
+                      bool argmax) {
     if (variant < 0 || variant > 1 || (uint32_t)variant >= mfma_variants(elem, dch)) variant = 0;
     const int i = mfma_tile_index(dch);
     if (i < 0)
@@ -1554,7 +1477,9 @@ void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_
                                            "(fp8) are not supported by the MFMA scan");
     const MfmaTiles &t = kMfmaTiles[i];
     int nt;
-    if (elem == kElemFP8)
+    if (argmax)
+        nt = elem == kElemBF16 ? t.am_bf_nt : t.am_f32_nt;
+    else if (elem == kElemFP8)
         nt = variant ? t.f8_nt1 : t.f8_nt;
     else if (elem == kElemBF16)
         nt = variant ? t.bf_nt1 : t.bf_nt;

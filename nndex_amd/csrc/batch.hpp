@@ -42,8 +42,8 @@ struct MfmaScanArgs {
     // of the grid are split into this many pieces of the query chunks
     // (0 or 1: not split)
     uint32_t tail_split;
-    // modes 0 and 1: the batch's chunk image (launch_chunk_image for this
-    // element type, width and variant), which the scan copies into LDS;
+    // the batch's chunk image (launch_chunk_image for this element type,
+    // width and variant, or the argmax shape), which the scan copies into LDS;
     // `thresh_f` (mode 0) and `q_scale` (FP8) are then read a whole chunk at
     // a time, so both hold nq rounded up to the chunk's query count (the
     // image's padding entries set by launch_chunk_image)
@@ -87,9 +87,11 @@ uint32_t mfma_variants(int elem, uint32_t dch);
 uint32_t mfma_block_rows(int elem, uint32_t dch, bool argmax = false, int variant = 0);
 void launch_mfma_scan(const MfmaScanArgs &a, int elem, uint32_t grid, hipStream_t s,
                       int variant = 0);
-// Chunk shape of the candidate / score scan for (element type, units per row,
-// variant): queries per chunk and unit rows per query (4 * steps >= dch).
-void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_t &ru);
+// Chunk shape of the candidate / score scan (or, with argmax, of the row
+// argmax) for (element type, units per row, variant): queries per chunk and
+// unit rows per query (4 * steps >= dch).
+void mfma_chunk_shape(int elem, uint32_t dch, int variant, uint32_t &qc, uint32_t &ru,
+                      bool argmax = false);
 // Largest chunk image of any shape for nq queries, in 16-byte units.
 inline size_t mfma_image_units(size_t nq) { return (nq + 64) * 128; }
 // Writes the chunk image of nq row-major queries (q_stride units apart, dch

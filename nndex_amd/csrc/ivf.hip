@@ -950,7 +950,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
     unsigned long long *rng = nullptr, *medoid = nullptr;
     uint32_t *sorted_keys = nullptr, *rows_iota = nullptr, *counts = nullptr, *recheck = nullptr,
              *recheck_count = nullptr;
-    uint4 *cbf16 = nullptr;
+    uint4 *cbf16 = nullptr, *cimg = nullptr;
     unsigned long long *cerr = nullptr;
     float *tol = nullptr;
     void *sort_tmp = nullptr;
@@ -959,6 +959,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         for (void *p : {(void *)partial, (void *)colpart, (void *)min_dist, (void *)mean,
                         (void *)rng, (void *)medoid, (void *)sorted_keys, (void *)rows_iota,
                         (void *)counts, (void *)recheck, (void *)recheck_count, (void *)cbf16,
+                        (void *)cimg,
                         (void *)cerr, (void *)tol, (void *)lastb, sort_tmp})
             if (p) (void)hipFree(p);
     };
@@ -978,6 +979,7 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
         NNDEX_HIP_CHECK(hipMalloc(&tol, sizeof(float)));
         NNDEX_HIP_CHECK(hipMalloc(&lastb, d4 * sizeof(float4)));
         if (bf16) NNDEX_HIP_CHECK(hipMalloc(&cbf16, (size_t)nlist * d8 * sizeof(uint4)));
+        if (!wide) NNDEX_HIP_CHECK(hipMalloc(&cimg, mfma_image_units(nlist) * sizeof(uint4)));
         uint32_t bits = 1;
         while ((1u << bits) < nlist) ++bits;
         NNDEX_HIP_CHECK(rocprim::radix_sort_pairs(sort_tmp, sort_bytes, ivf.assign, sorted_keys,
@@ -1084,6 +1086,14 @@ void ivf_build(IvfIndex &ivf, const float4 *x, const uint4 *xbf16, uint32_t rows
                 m.q_stride = m.dch;
                 m.nq = nlist;
                 m.mode = 2;
+                // this iteration's centroids as the scan's chunk image
+                {
+                    uint32_t qc = 0, ru = 0;
+                    mfma_chunk_shape(bf16, m.dch, 0, qc, ru, true);
+                    launch_chunk_image(m.q, m.q_stride, nlist, m.dch, qc, ru, cimg, nullptr,
+                                       nullptr, s);
+                    m.qimg = cimg;
+                }
                 m.assign = ivf.assign;
                 m.tol = tol;
                 m.recheck = recheck;

@@ -153,7 +153,7 @@ def test_ivf_scan_indices_stay_in_bounds(orc, rows, dims, k, nprobe, nq):
 # ---------------------------------------------------------------------------
 
 def _nbuf(maxs, nt, scaled, nw=8, queue=True):
-    buf = (maxs * 4 * 16 * nt + 8 * nt + (4 * nt if scaled else 0)) * 16
+    buf = (maxs * 4 * 16 * nt + 4 * nt + (4 * nt if scaled else 0)) * 16
     q = nw * 128 * 12 if queue else 0
     lds = 160 * 1024
     return 4 if 4 * buf + q <= lds else (3 if 3 * buf + q <= lds else 2)
@@ -163,10 +163,11 @@ def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2):
     qc = 16 * nt
     ru = 4 * maxs
     q_units = qc * ru
-    tu, su = qc // 2, (qc // 4 if scaled else 0)
+    tu, su = qc // 4, (qc // 4 if scaled else 0)
     buf_units = q_units + tu + su
-    nb = _nbuf(maxs, nt, scaled, nw)
-    assert (nb * buf_units + nw * 128 * 12 // 16) * 16 <= 160 * 1024
+    am = mode == 2
+    nb = _nbuf(maxs, nt, scaled, nw, queue=not am)
+    assert (nb * buf_units + (0 if am else nw * 128 * 12 // 16)) * 16 <= 160 * 1024
     ni = nt * maxs
     nch_all = -(-nq // qc)
     image_units = nch_all * qc * ru
@@ -265,6 +266,9 @@ def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2):
     (100, 8, 2, 0, False, 1),     # tail shorter than the ring
     (33, 2, 4, 1, False, 1),
     (4096, 32, 1, 0, False, 8),
+    (4096, 8, 2, 2, False, 1),    # row argmax over 4,096 centroids at 128 dims
+    (89, 4, 4, 2, False, 1),      # row argmax, 89 centroids at 64 dims
+    (300, 12, 4, 2, False, 1),
 ])
 def test_mfma_scan_ring_schedule(nq, maxs, nt, mode, scaled, qsplit):
     ring_replay(nq, maxs, nt, mode, scaled, qsplit=qsplit)
