@@ -57,6 +57,8 @@ struct F32E {
     // the skewed epilogue only pays where MFMA steps are short (16-bit and
     // 8-bit forms); four f32 MFMAs per step already hide the epilogue
     static constexpr bool SKEW = false;
+    // the mid-chunk staggered barrier (below) was measured on the BF16 form
+    static constexpr bool STAGGER = false;
     static __device__ inline void mma2(uint4, uint4, uint4, uint4, uint32_t, uint32_t,
                                        v4f (&)[NACC]) {}
 };
@@ -68,7 +70,7 @@ struct BF16E {
                                                        __builtin_bit_cast(v8bf, b), c[0], 0, 0, 0);
     }
     static __device__ inline float total(const v4f (&c)[NACC], int v) { return c[0][v]; }
-    static constexpr bool SCALED = false, PAIR = false, SKEW = true;
+    static constexpr bool SCALED = false, PAIR = false, SKEW = true, STAGGER = true;
     static __device__ inline void mma2(uint4, uint4, uint4, uint4, uint32_t, uint32_t,
                                        v4f (&)[NACC]) {}
 };
@@ -81,7 +83,8 @@ struct FP8E {
     // query; the same for every K block); the epilogue multiplies by the
     // scales' mantissas.
     static constexpr int NACC = 1;
-    static constexpr bool SCALED = true, PAIR = true, SKEW = true;
+    // (the staggered barrier measured no gain here and costs registers)
+    static constexpr bool SCALED = true, PAIR = true, SKEW = true, STAGGER = false;
     static __device__ inline void mma(uint4, uint4, v4f (&)[NACC]) {}
     static __device__ inline void mma2(uint4 a0, uint4 a1, uint4 b0, uint4 b1, uint32_t sa,
                                        uint32_t sb, v4f (&c)[NACC]) {
@@ -670,9 +673,23 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             }
         }
     };
-    // Half of the waves (one of each SIMD's pair) run the mode 0 epilogue one
-    // chunk late, so one wave's epilogue overlaps the other's MFMAs.
-    const bool late = E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
+    // BF16 candidate and score scans with three or more ring buffers: the
+    // chunk barrier sits inside the MFMA stream, at a different step group in
+    // the two halves of the block (one wave of each SIMD's pair right after
+    // the first group, the other after the last), so whichever wave waits at
+    // it, its SIMD's other wave still has MFMAs to issue.  The barrier of
+    // chunk ch follows every wave's last read of chunk ch - 1 and the landing
+    // of chunk ch + 1's copies; after it a wave copies chunk ch + NB - 1 into
+    // chunk ch - 1's buffer, which is neither of the two buffers read until
+    // the next barrier (ch, ch + 1).
+    constexpr bool MIDBAR = E::STAGGER && !AM && NB >= 3;
+    constexpr int SW = E::PAIR ? 2 : 1;
+    constexpr int NG = MAXS / SW;
+    const bool bar_last = wid >= NW / 2;  // barrier after the last step group, else the first
+    // Two-buffer rings: half of the waves (one of each SIMD's pair) run the
+    // mode 0 epilogue one chunk late, so one wave's epilogue overlaps the
+    // other's MFMAs.
+    const bool late = !MIDBAR && E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
     v4f accp[MT][NT][E::NACC];
     float thp[NT], fqp[NT];
     for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -781,9 +798,12 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             // An MMA covers SW steps (SW = 2 for the K=128 FP8 form).  Unit
             // row u = 4 S + g of step S sits at lane slot (l16 ^ u) & 15 =
             // l16 ^ (4 (S & 3) + g): four base pointers, constant offsets.
-            constexpr int SW = E::PAIR ? 2 : 1;
-            constexpr int NG = MAXS / SW;
             constexpr int PD = NG > 2 ? 2 : 1;
+            // mid-chunk form: chunks whose copies were issued past this one;
+            // the barrier waits for chunk ch + 1's copies, while those of the
+            // chunks after it (issued later, in order) may stay in flight
+            const uint32_t ahead_mid = ch + NB - 2 < ch1 - 1 ? NB - 2 : ch1 - 1 - ch;
+            const uint32_t wait_n = ahead_mid >= 2 ? (ahead_mid - 1) * my_cnt : 0;
             const uint4 *sb[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
@@ -816,6 +836,15 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                             E::mma(xr[m][st], qc[0][t], acc[m][t]);
                     }
                 __builtin_amdgcn_sched_barrier(0);
+                if (MIDBAR && (grp == 0 || grp == NG - 1)) {
+                    // wave-uniform: one barrier per wave per chunk
+                    if (bar_last == (grp == NG - 1)) {
+                        wait_vm_dyn(wait_n);
+                        __syncthreads();
+                        if (ch + NB - 1 < ch1) dma_chunk(ch + NB - 1, (jj + NB - 1) % NB);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
             }
             if (!AM && a.mode == 0) {
                 if (late) {
@@ -868,15 +897,15 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 }
             }
-            // chunk ch + NB - 1 into the buffer chunk ch - 1 used (every wave
-            // passed the barrier after reading it); then the copies of chunk
-            // ch + 1 must have landed before the barrier, while those of the
-            // chunks after it (ahead - 1 of them, each my_cnt copies, issued
-            // later in order) may stay in flight
-            if (ch + NB - 1 < ch1) dma_chunk(ch + NB - 1, (jj + NB - 1) % NB);
-            const uint32_t ahead = ch + NB - 1 < ch1 ? NB - 1 : ch1 - 1 - ch;
-            wait_vm_dyn(ahead >= 2 ? (ahead - 1) * my_cnt : 0);
-            __syncthreads();
+            if constexpr (!MIDBAR) {
+                // two buffers: chunk ch + 1 into the buffer chunk ch - 1 used
+                // (every wave passed the barrier after reading it); its copies
+                // must land before the barrier at the end of this chunk
+                if (ch + NB - 1 < ch1) dma_chunk(ch + NB - 1, (jj + NB - 1) % NB);
+                const uint32_t ahead = ch + NB - 1 < ch1 ? NB - 1 : ch1 - 1 - ch;
+                wait_vm_dyn(ahead >= 2 ? (ahead - 1) * my_cnt : 0);
+                __syncthreads();
+            }
         }
         if constexpr (!AM) {
             if (late) epi0(accp, thp, fqp, fr, ch1 - 1, bt);
@@ -1414,7 +1443,7 @@ static constexpr MfmaTiles kMfmaTiles[] = {
     {4, 4, 32, 4, 32, 2, 2, 32, 2, 16, 2, 4, 32, 4},
     {6, 2, 32, 2, 32, 2, 4, 32, 4, 16, 2, 2, 32, 2},
     {8, 2, 32, 2, 16, 2, 4, 32, 4, 32, 2, 2, 32, 2},
-    {12, 2, 32, 2, 32, 2, 1, 16, 4, 16, 2, 2, 16, 4},
+    {12, 2, 32, 4, 32, 2, 1, 32, 2, 16, 2, 2, 16, 4},
     {16, 2, 32, 1, 32, 1, 1, 16, 2, 16, 2, 2, 16, 2},
     {24, 2, 16, 2, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},
     {32, 1, 16, 1, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},

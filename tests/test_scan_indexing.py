@@ -159,7 +159,7 @@ def _nbuf(maxs, nt, scaled, nw=8, queue=True):
     return 4 if 4 * buf + q <= lds else (3 if 3 * buf + q <= lds else 2)
 
 
-def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2):
+def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2, midbar=False):
     qc = 16 * nt
     ru = 4 * maxs
     q_units = qc * ru
@@ -231,6 +231,9 @@ def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2):
                 if ch0 + j < ch1:
                     assert not pending.get(j)
                     landed[j] = ch0 + j
+            if midbar:
+                _mid_chunks(ch0, ch1, nb, nw, my_cnt, issue, wait, inflight, pending, landed)
+                continue
             for ch in range(ch0, ch1):
                 jj = ch - ch0
                 cur = jj % nb
@@ -253,6 +256,67 @@ def ring_replay(nq, maxs, nt, mode, scaled, nw=8, qsplit=1, tiles=2):
                     assert not pending.get(b), (nxt, b, pending.get(b))
                     landed[b] = nxt
             assert all(not inflight[w] for w in range(nw))
+
+
+def _mid_chunks(ch0, ch1, nb, nw, my_cnt, issue, wait, inflight, pending, landed):
+    """The BF16 scans' mid-chunk schedule (three or more buffers): chunk ch's
+    barrier sits after step group 0 in waves 0 .. nw/2 - 1 and after the last
+    group in the others; before it a wave waits until chunk ch + 1's copies
+    have landed (at most (ahead_mid - 1) * my_cnt of its copies in flight,
+    ahead_mid = min(nb - 2, ch1 - 1 - ch)), after it the wave copies chunk
+    ch + nb - 1 into chunk ch - 1's buffer.  Between the barriers of chunks
+    ch - 1 and ch the waves, in any order, read the rest of chunk ch - 1 and
+    the first part of chunk ch and issue chunk ch + nb - 2's copies: the
+    buffers read and the buffers written or still receiving copies in that
+    span must be disjoint, and chunk ch must have landed before it."""
+    assert nb >= 3
+    for ch in range(ch0, ch1):
+        jj = ch - ch0
+        cur = jj % nb
+        # span from the previous barrier (or the prologue's) to this one
+        reads = {cur} | ({(jj - 1) % nb} if ch > ch0 else set())
+        assert landed.get(cur) == ch, (ch, cur, landed)
+        if ch > ch0:
+            prev = ch - 1
+            if prev + nb - 1 < ch1:
+                tgt = (jj - 1 + nb - 1) % nb
+                assert tgt not in reads, (ch, tgt, reads)
+                landed.pop(tgt, None)
+                for w in range(nw):
+                    assert issue(w, prev + nb - 1, tgt) == my_cnt[w]
+        # copies in flight during the span land only in buffers not read in it
+        for buf, waves in pending.items():
+            if waves:
+                assert buf not in reads, (ch, buf, reads)
+        # this chunk's barrier: every wave waits for chunk ch + 1's copies
+        ahead_mid = min(nb - 2, ch1 - 1 - ch)
+        for w in range(nw):
+            wait(w, (ahead_mid - 1) * my_cnt[w] if ahead_mid >= 2 else 0)
+        nxt = ch + 1
+        if nxt < ch1:
+            b = (jj + 1) % nb
+            assert not pending.get(b), (nxt, b, pending.get(b))
+            landed[b] = nxt
+    # after the last chunk's barrier: the rest of it is read and no copy is
+    # issued (chunk ch1 - 1 + nb - 1 is past the range); nothing in flight
+    assert all(not inflight[w] for w in range(nw))
+
+
+@pytest.mark.parametrize("nq,maxs,nt,mode", [
+    (2048, 12, 4, 0),   # BF16 384 dims, 64-query chunks: 3 buffers
+    (2048, 12, 4, 1),
+    (2048, 12, 2, 0),   # 32-query chunks: 4 buffers
+    (1024, 24, 2, 0),   # BF16 768 dims: 3 buffers
+    (1024, 24, 1, 1),
+    (17, 4, 4, 0),      # one partial chunk
+    (100, 8, 2, 0),     # tail shorter than the ring
+    (130, 12, 4, 0),    # three chunks over three buffers
+    (4096, 16, 2, 1),
+])
+def test_mfma_scan_mid_chunk_barrier_schedule(nq, maxs, nt, mode):
+    if _nbuf(maxs, nt, False) < 3:
+        pytest.skip("two-buffer rings keep the end-of-chunk barrier")
+    ring_replay(nq, maxs, nt, mode, False, midbar=True)
 
 
 @pytest.mark.parametrize("nq,maxs,nt,mode,scaled,qsplit", [
