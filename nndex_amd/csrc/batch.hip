@@ -1445,7 +1445,7 @@ static constexpr MfmaTiles kMfmaTiles[] = {
     {8, 2, 32, 2, 16, 2, 4, 32, 4, 32, 2, 2, 32, 2},
     {12, 2, 32, 4, 32, 2, 1, 32, 2, 16, 2, 2, 16, 4},
     {16, 2, 32, 1, 32, 1, 1, 16, 2, 16, 2, 2, 16, 2},
-    {24, 2, 16, 2, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},
+    {24, 2, 32, 1, 16, 1, 1, 16, 2, 16, 1, 1, 16, 1},
     {32, 1, 16, 1, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},
 };
 constexpr int kMfmaTileCount = sizeof(kMfmaTiles) / sizeof(kMfmaTiles[0]);
