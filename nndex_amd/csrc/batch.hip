@@ -1443,7 +1443,7 @@ static constexpr MfmaTiles kMfmaTiles[] = {
     {4, 4, 32, 4, 32, 2, 2, 32, 2, 16, 2, 4, 32, 4},
     {6, 2, 32, 2, 32, 2, 4, 32, 4, 16, 2, 2, 32, 2},
     {8, 2, 32, 2, 16, 2, 4, 32, 4, 32, 2, 2, 32, 2},
-    {12, 2, 32, 4, 32, 2, 1, 32, 2, 16, 2, 2, 16, 4},
+    {12, 2, 48, 2, 32, 2, 1, 32, 4, 16, 2, 2, 16, 4},
     {16, 2, 32, 1, 32, 1, 1, 16, 2, 16, 2, 2, 16, 2},
     {24, 2, 32, 1, 16, 1, 1, 16, 2, 16, 1, 1, 16, 1},
     {32, 1, 16, 1, 16, 1, 1, 16, 1, 16, 1, 1, 16, 1},
