@@ -837,7 +837,11 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 __builtin_amdgcn_sched_barrier(0);
                 if (MIDBAR && (grp == 0 || grp == NG - 1)) {
-                    // wave-uniform: one barrier per wave per chunk
+                    // wave-uniform: one barrier per wave per chunk.  The two
+                    // halves of the block reach it at different instructions;
+                    // s_barrier counts one arrival per wave wherever the wave
+                    // executes it, so every chunk's barrier still joins all
+                    // NW waves (every wave runs the same chunk count)
                     if (bar_last == (grp == NG - 1)) {
                         wait_vm_dyn(wait_n);
                         __syncthreads();
