@@ -240,6 +240,29 @@ def test_batch_tile_variants_agree_and_autotune(dims):
             assert gi[i].tolist() == ei[i].tolist() and gs[i].tolist() == es[i].tolist()
 
 
+@pytest.mark.parametrize("rows,dims", [(150_000, 384), (100_000, 768)])
+def test_bf16_scan_many_tiles_per_block(rows, dims):
+    """More row tiles than blocks (one block per CU loops over tiles, the
+    last round split into query-chunk pieces): the BF16 scans with the chunk
+    barrier inside the MFMA stream, at either tile shape, return for every
+    certified query exactly the fused FP32 scan's result."""
+    _case("bf16_many_tiles", rows=rows, dims=dims)
+    base, queries = synth.mixture_workload(rows=rows, dims=dims, n_queries=200)
+    b = BackendIndex(base)
+    ds = DeviceSearch(b.h, 200, 64)
+    L = _lib.lib()
+    ei, es = exact_device(b, ds, queries, 10)
+    for v in (0, 1):
+        check(L.nndex_gpu_index_set_batch_tiles(b.h, 1, v))
+        gi, gs, fl = batch_device(b, ds, queries, 10, 1)
+        ok = fl == 0
+        assert ok.mean() > 0.9, (v, ok.mean())
+        for i in np.flatnonzero(ok):
+            assert gi[i].tolist() == ei[i].tolist(), (v, i)
+            assert gs[i].tolist() == es[i].tolist(), (v, i)
+    check(L.nndex_gpu_index_set_batch_tiles(b.h, 1, 0))
+
+
 @pytest.mark.parametrize("dims", [600, 768, 1000])
 def test_wide_rows_tile_variants(dims):
     """Above 512 dims (BF16 and FP8 scoring only) with either tile variant:
