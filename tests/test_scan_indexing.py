@@ -302,21 +302,24 @@ def _mid_chunks(ch0, ch1, nb, nw, my_cnt, issue, wait, inflight, pending, landed
     assert all(not inflight[w] for w in range(nw))
 
 
-@pytest.mark.parametrize("nq,maxs,nt,mode", [
-    (2048, 12, 4, 0),   # BF16 384 dims, 64-query chunks: 3 buffers
-    (2048, 12, 4, 1),
-    (2048, 12, 2, 0),   # 32-query chunks: 4 buffers
-    (1024, 24, 2, 0),   # BF16 768 dims: 3 buffers
-    (1024, 24, 1, 1),
-    (17, 4, 4, 0),      # one partial chunk
-    (100, 8, 2, 0),     # tail shorter than the ring
-    (130, 12, 4, 0),    # three chunks over three buffers
-    (4096, 16, 2, 1),
+@pytest.mark.parametrize("nq,maxs,nt,mode,qsplit", [
+    (2048, 12, 2, 0, 1),   # BF16 384 dims, 32-query chunks: 4 buffers
+    (2048, 12, 2, 1, 2),   # its sample scan, queries split over 2 block rows
+    (2048, 12, 4, 0, 1),   # 64-query chunks: 3 buffers
+    (2048, 12, 4, 1, 3),
+    (1024, 24, 1, 0, 1),   # BF16 768 dims, 16-query chunks: 4 buffers
+    (1024, 24, 1, 1, 5),
+    (1024, 24, 2, 0, 1),   # 32-query chunks: 3 buffers
+    (17, 4, 4, 0, 1),      # one partial chunk
+    (100, 8, 2, 0, 1),     # tail shorter than the ring
+    (130, 12, 4, 0, 1),    # three chunks over three buffers
+    (4096, 16, 2, 1, 8),
+    (4096, 32, 1, 0, 1),   # BF16 1024 dims
 ])
-def test_mfma_scan_mid_chunk_barrier_schedule(nq, maxs, nt, mode):
+def test_mfma_scan_mid_chunk_barrier_schedule(nq, maxs, nt, mode, qsplit):
     if _nbuf(maxs, nt, False) < 3:
         pytest.skip("two-buffer rings keep the end-of-chunk barrier")
-    ring_replay(nq, maxs, nt, mode, False, midbar=True)
+    ring_replay(nq, maxs, nt, mode, False, qsplit=qsplit, midbar=True)
 
 
 @pytest.mark.parametrize("nq,maxs,nt,mode,scaled,qsplit", [
