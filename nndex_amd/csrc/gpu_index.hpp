@@ -123,7 +123,7 @@ struct BatchPlan {
     uint32_t stride = 1;
     size_t rows_s = 0;
     size_t capq = 0;
-    size_t group = 2048;
+    size_t group = 4096;
     // device bytes of the workspace buffers for nq queries
     size_t workspace_bytes(size_t nq, size_t pdims) const;
 };
