@@ -475,8 +475,10 @@ __global__ __launch_bounds__(256) void prep_queries_kernel(const float *__restri
 // X-stationary MFMA scan.  NW waves per block, RW rows per wave (multiple of
 // 16), MAXS 16-byte units per lane per row (units per row <= 4 * MAXS; lane
 // group g = lane/16 owns units 4s+g), NT 16-query MFMA tiles per query chunk.
-// Mode 0 appends candidates, mode 1 writes scores; the AM variant (mode 2)
-// keeps each row's two best queries (centroids) and writes the row argmax.
+// Mode 0 appends candidates, mode 1 writes scores, mode 3 writes the largest
+// score of each group of RW / 4 rows (a lane's rows of the wave's tiles: group
+// (tile NW + wave) 4 + lane group); the AM variant (mode 2) keeps each row's
+// two best queries (centroids) and writes the row argmax.
 // Query chunks come from the batch's chunk image (chunk_image_kernel) by LDS
 // copies issued NB - 1 chunks ahead into a ring of NB buffers, so a chunk's
 // copies have NB - 2 chunks of MFMA work to land in and no registers hold
@@ -507,8 +509,12 @@ constexpr size_t mfma_lds_bytes() {
            sizeof(uint4);
 }
 
-template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT, bool CAND>
 __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
+    // CAND: the candidate scan (mode 0); otherwise modes 1 and 3 (scores,
+    // group maxima) or the row argmax (AM).  Separate instances keep each
+    // form's registers free of the other's epilogue.
+    static_assert(!(AM && CAND), "the row argmax appends no candidates");
     static_assert(!(AM && E::SCALED), "the row argmax takes unscaled scores");
     constexpr int MT = RW / 16;          // 16-row MFMA tiles per wave
     constexpr uint32_t QC = 16 * NT;     // queries per chunk
@@ -582,7 +588,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // scales (scaled form), copy i issued by wave i % NW; my_cnt = this
     // wave's.
     constexpr uint32_t NI = (uint32_t)(NT * MAXS);
-    const bool stage_th = a.mode == 0;
+    constexpr bool stage_th = CAND;
     const uint32_t my_cnt = (NI > wid ? (NI - wid + NW - 1) / NW : 0) +
                             ((stage_th && wid == NI % NW) ? 1u : 0u) +
                             ((E::SCALED && wid == (NI + 1) % NW) ? 1u : 0u);
@@ -689,7 +695,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
     // Two-buffer rings: half of the waves (one of each SIMD's pair) run the
     // mode 0 epilogue one chunk late, so one wave's epilogue overlaps the
     // other's MFMAs.
-    const bool late = !MIDBAR && E::SKEW && !AM && a.mode == 0 && NW >= 8 && wid >= NW / 2;
+    const bool late = !MIDBAR && E::SKEW && CAND && NW >= 8 && wid >= NW / 2;
     v4f accp[MT][NT][E::NACC];
     float thp[NT], fqp[NT];
     for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -753,6 +759,14 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 }
             }
         }
+        // the lane's result rows (16 m + 4 g + v of each tile) inside the
+        // index, as bits m * 4 + v (mode 3 leaves the others out)
+        uint32_t row_bits = 0;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (bt * block_rows + wid * RW + m * 16 + 4 * g + v < a.rows) row_bits |= 1u << (m * 4 + v);
         // the stationary rows (and scales) have landed: a wait the compiler
         // sees, so it adds none for them inside the chunk loop (there it
         // would also wait for the copies in flight)
@@ -778,7 +792,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                 thv[t] = 0.f;
                 fqv[t] = 1.f;
                 sbq[t] = 127u;
-                if (!AM && a.mode == 0) thv[t] = thfl(cur)[t * 16 + l16];
+                if constexpr (CAND) thv[t] = thfl(cur)[t * 16 + l16];
                 if constexpr (E::SCALED) {
                     const float sq = qsl(cur)[t * 16 + l16];
                     sbq[t] = e8m0_of(sq);
@@ -850,7 +864,7 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
                     }
                 }
             }
-            if (!AM && a.mode == 0) {
+            if constexpr (CAND) {
                 if (late) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
@@ -869,8 +883,25 @@ __global__ __launch_bounds__(NW * 64, 2) void mfma_scan_kernel(MfmaScanArgs a) {
             // each tile, query 16t+l16 ----
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                if (!AM && a.mode == 0) break;
+                if constexpr (CAND) break;
                 const uint32_t qi = ch * QC + t * 16 + l16;
+                if (!AM && a.mode == 3) {
+                    // group maxima: the lane's 4 MT rows (16 m + 4 g + v of
+                    // the wave's tiles) form group (bt NW + wid) 4 + g; rows
+                    // past the index are left out (a group of padding rows
+                    // gives -inf)
+                    float mx = -__builtin_inff();
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            float sv = E::total(acc[m][t], v);
+                            if constexpr (E::SCALED) sv = (sv * fr[m][v]) * fqv[t];
+                            if ((row_bits >> (m * 4 + v)) & 1u) mx = __builtin_fmaxf(mx, sv);
+                        }
+                    if (qi < a.nq) a.out[(size_t)qi * a.ldo + (bt * NW + wid) * 4 + g] = mx;
+                    continue;
+                }
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const uint32_t row0 = bt * block_rows + wid * RW + m * 16 + 4 * g;
@@ -1399,13 +1430,13 @@ void launch_mfma_tile(int elem, const void *a, const void *b, uint32_t calls, fl
     NNDEX_HIP_CHECK(hipGetLastError());
 }
 
-template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT>
+template <typename E, int NW, int RW, int MAXS, int NT, bool AM, bool SPLIT, bool CAND>
 static void launch_mfma_split(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     // the ring of query buffers, then the waves' candidate queues
     constexpr size_t lds = mfma_lds_bytes<E, NW, MAXS, NT, AM>();
     static_assert(lds <= 160 * 1024, "LDS of the MFMA scan");
     if (!a.qimg) throw Error(NNDEX_ERR_INVALID_ARGUMENT, "the MFMA scan needs a chunk image");
-    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM, SPLIT>;
+    auto fn = mfma_scan_kernel<E, NW, RW, MAXS, NT, AM, SPLIT, CAND>;
     // once per instantiation (also keeps the call out of graph captures);
     // serving executors launch from several threads, so the first call sets
     // it and every other waits until it is set
@@ -1423,9 +1454,15 @@ static void launch_mfma_split(const MfmaScanArgs &a, uint32_t grid, hipStream_t 
 template <typename E, int NW, int RW, int MAXS, int NT, bool AM = false>
 static void launch_mfma_t(const MfmaScanArgs &a, uint32_t grid, hipStream_t s) {
     if constexpr (!AM) {
-        if (a.qsplit > 1) return launch_mfma_split<E, NW, RW, MAXS, NT, AM, true>(a, grid, s);
+        if (a.mode != 0 && a.mode != 1 && a.mode != 3)
+            throw Error(NNDEX_ERR_INVALID_ARGUMENT, "MFMA scan mode");
+        if (a.mode == 0) {
+            if (a.qsplit > 1) return launch_mfma_split<E, NW, RW, MAXS, NT, AM, true, true>(a, grid, s);
+            return launch_mfma_split<E, NW, RW, MAXS, NT, AM, false, true>(a, grid, s);
+        }
+        if (a.qsplit > 1) return launch_mfma_split<E, NW, RW, MAXS, NT, AM, true, false>(a, grid, s);
     }
-    launch_mfma_split<E, NW, RW, MAXS, NT, AM, false>(a, grid, s);
+    launch_mfma_split<E, NW, RW, MAXS, NT, AM, false, false>(a, grid, s);
 }
 
 // Tile shapes per step count (a row of `dch` 16-byte units runs the smallest

@@ -20,7 +20,8 @@ struct MfmaScanArgs {
     const float *row_scale;    // fp8: per-row scale [rows_padded]
     const float *q_scale;      // fp8: per-query scale [nq]
     uint32_t nq;
-    uint32_t mode;             // 0: append candidates, 1: write scores, 2: row argmax
+    uint32_t mode;             // 0: append candidates, 1: write scores, 2: row argmax,
+                               // 3: write each group's largest score (out[nq][ldo])
     const unsigned long long *thresh;  // [nq] key thresholds (register staging)
     const float *thresh_f;     // [nq] their float floors (mode 0, padded: see qimg)
     uint64_t *cand;            // [nq][capq] candidate keys (mode 0)

@@ -438,6 +438,18 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
         a.out = ws.scores;
         a.ldo = static_cast<uint32_t>(sample_ld(smp.rows));
         const uint32_t br = mfma_block_rows(elem, dch, false, variant);
+        // A large sample writes only each group's largest score (32 groups
+        // of br / 32 rows per block tile): the jt-th best group maximum is
+        // still a score that at least jt sample rows reach (jt groups, one
+        // row each), and almost always the jt-th best row itself, since the
+        // best sample rows seldom share a group; the scan writes and the
+        // threshold reads br / 32 times fewer scores.
+        uint32_t n_sample = smp.rows;
+        const uint32_t n_groups = static_cast<uint32_t>((a.rows_padded + br - 1) / br) * 32;
+        if (smp.rows / (br / 32) >= 16 * plan.jt && n_groups <= a.ldo) {
+            a.mode = 3;
+            n_sample = n_groups;
+        }
         // one 8-wave block per CU (register-bound); a scan with fewer row
         // tiles than CUs also splits the queries over blocks (grid y),
         // choosing the split that finishes in the fewest block waves per
@@ -476,7 +488,7 @@ void GpuIndex::search_batch_mfma(Workspace &ws, const float *dq, size_t nq, size
             return blocks < cap ? blocks : cap;
         };
         launch_mfma_scan(a, elem, grid_for(a.rows_padded), s, variant);
-        launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), smp.rows, a.ldo,
+        launch_sample_threshold(ws.scores, static_cast<uint32_t>(n), n_sample, a.ldo,
                                 static_cast<uint32_t>(plan.jt), ws.delta, ws.thresh, ws.thresh_f, s);
         // stage 2: filter scan over every row
         launch_zero_u32(ws.cand_count, n, s);
