@@ -203,6 +203,7 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     __shared__ uint64_t slots[4][QB][CW];
     __shared__ uint32_t wave_count[4][QB];
     __shared__ uint32_t is_last;
+    __shared__ uint32_t pool_n, sel_hi;
 
     const uint32_t lane = __lane_id();
     const uint32_t wid = threadIdx.x >> 6;
@@ -211,10 +212,10 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     const float4 *__restrict__ X = a.x;
     const_as<f32x4> *Q = as_constant(reinterpret_cast<const f32x4 *>(a.q));
 
-    // Per-wave thresholds only: the device-wide one costs more than it
-    // prunes here, since reading it back waits for every outstanding row
-    // load of the wave (vector memory counters complete in order).  Blocks
-    // publish all their keys (the theta slots stay 0) to the final merge.
+    // Per-wave thresholds only: a device-wide one costs more than it prunes
+    // here, since reading it back waits for every outstanding row load of
+    // the wave (vector memory counters complete in order).  Blocks publish
+    // their merged lists to the final merge.
     WaveTopK tk[QB];
 #pragma unroll
     for (int q = 0; q < QB; ++q) tk[q].init(slots[wid][q], CW, a.k, nullptr, a.k_out);
@@ -345,24 +346,12 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
                 }
             }
             tk[q].compact();
-            // Publish only the keys at or above the device-wide threshold as
-            // it stands now (it only grows), with their count.
-            const unsigned long long th =
-                __hip_atomic_load(a.theta + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t live = 0;
-            for (uint32_t i0 = 0; i0 < tk[q].count; i0 += 64) {
-                const uint32_t i = i0 + lane;
-                live += __popcll(__ballot(i < tk[q].count && slots[0][q][i] >= th));
-            }
-            // one dense list per query: the block reserves its range with
-            // one atomic, so the final merge reads no empty slots
-            uint32_t base = 0;
-            if (lane == 0 && live)
-                base = __hip_atomic_fetch_add(a.part_count + q, live, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-            base = __shfl(base, 0);
-            uint64_t *dst = a.part + (size_t)q * gridDim.x * a.k + base;
-            for (uint32_t i = lane; i < live; i += 64) dst[i] = slots[0][q][i];
+            // Publish the block's list (sorted, at most k keys) in the
+            // block's own k slots of the query's table, empty slots as key 0
+            // (no key is 0), so the final merge finds each block's largest
+            // key at the start of its slots
+            uint64_t *dst = a.part + ((size_t)q * gridDim.x + blockIdx.x) * a.k;
+            for (uint32_t i = lane; i < a.k; i += 64) dst[i] = i < tk[q].count ? slots[0][q][i] : 0;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         uint32_t ticket = 0;
@@ -376,64 +365,125 @@ __global__ __launch_bounds__(256) void exact_scan_kernel(ScanArgs a) {
     if (!is_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-    // Final merge of the published keys per query (>= final threshold), in
-    // independent batches of 8 loads per lane before they are offered.
+    // Final merge, one query at a time.  (1) S = the k_out-th largest score
+    // among the blocks' largest keys, found bit by bit with ballots: at least
+    // k_out rows score S or more, so a row scoring below S is neither in the
+    // result nor tied with its last score.  (2) Every published key scoring
+    // at least S goes to one LDS pool (the slots, free now).  (3) Wave 0
+    // selects the result from the pool as a wave's list would.  A pool too
+    // small for the keys that pass (ties or large k) falls back to every
+    // wave offering its share of the table; more than 512 blocks skip (1).
     const uint32_t nb = gridDim.x;
+    uint64_t *pool = &slots[0][0][0];
+    constexpr uint32_t POOL = 4 * QB * CW;
+    constexpr int MB = 8;   // block maxima per lane in (1)
+    constexpr int KB = 16;  // table keys per lane per batch
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-        const unsigned long long theta =
-            __hip_atomic_load(a.theta + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tk[q].init(slots[wid][q], CW, a.k, nullptr, a.k_out);
-        const uint64_t *src = a.part + (size_t)q * nb * a.k;
-        const uint32_t total =
-            __hip_atomic_load(a.part_count + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        constexpr int KB = 8;
-        for (uint32_t j0 = wid * 64 * KB; j0 < total; j0 += 256 * KB) {
-            uint64_t key[KB];
-            bool ok[KB];
+        const uint64_t *tab = a.part + (size_t)q * nb * a.k;
+        const uint32_t nslots = nb * a.k;
+        // the first batch of table keys is in flight during (1)
+        uint64_t key[KB];
 #pragma unroll
-            for (int u = 0; u < KB; ++u) {
-                const uint32_t j = j0 + u * 64 + lane;
-                ok[u] = j < total;
-                key[u] = ok[u] ? src[j] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < KB; ++u) tk[q].offer(key[u], ok[u] && key[u] >= theta);
+        for (int u = 0; u < KB; ++u) {
+            const uint32_t j = (u * 4 + wid) * 64 + lane;
+            key[u] = j < nslots ? tab[j] : 0;
         }
-        tk[q].compact();
-        if (lane == 0) wave_count[wid][q] = tk[q].count;
-    }
-    __syncthreads();
-    if (wid == 0) {
+        if (wid == 0) {
+            uint32_t sel = 0;  // wave-uniform
+            if (nb >= a.k_out && nb <= 64 * MB) {
+                uint32_t hi[MB];
 #pragma unroll
-        for (int q = 0; q < QB; ++q) {
-            for (uint32_t w = 1; w < 4; ++w) {
-                uint32_t n = wave_count[w][q];
-                for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-                    uint32_t i = i0 + lane;
-                    uint64_t key = i < n ? slots[w][q][i] : 0;
-                    tk[q].offer(key, i < n);
+                for (int i = 0; i < MB; ++i) {
+                    const uint32_t b = i * 64 + lane;
+                    hi[i] = b < nb ? static_cast<uint32_t>(tab[(size_t)b * a.k] >> 32) : 0;
+                }
+                for (uint32_t bit = 32; bit-- > 0;) {
+                    const uint32_t cand = sel | (1u << bit);
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int i = 0; i < MB; ++i) cnt += __popcll(__ballot(hi[i] >= cand));
+                    if (cnt >= a.k_out) sel = cand;
                 }
             }
-            tk[q].compact();
+            if (lane == 0) {
+                sel_hi = sel;
+                pool_n = 0;
+            }
+        }
+        __syncthreads();
+        const uint64_t floor_key = static_cast<uint64_t>(sel_hi) << 32;
+        for (uint32_t j0 = 0;; j0 += 256 * KB) {
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const uint32_t j = j0 + (u * 4 + wid) * 64 + lane;
+                const bool pass = j < nslots && key[u] != 0 && key[u] >= floor_key;
+                const uint64_t m = __ballot(pass);
+                if (m == 0) continue;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&pool_n, static_cast<uint32_t>(__popcll(m)));
+                base = __shfl(base, 0);
+                const uint32_t pos = base + __popcll(m & __lanemask_lt());
+                if (pass && pos < POOL) pool[pos] = key[u];
+            }
+            if (j0 + 256 * KB >= nslots) break;
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const uint32_t j = j0 + 256 * KB + (u * 4 + wid) * 64 + lane;
+                key[u] = j < nslots ? tab[j] : 0;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = pool_n;
+        WaveTopK fm;
+        if (n > POOL) {
+            // every wave offers its share of the table to its own list, then
+            // wave 0 merges the four
+            fm.init(slots[wid][q], CW, a.k, nullptr, a.k_out);
+            for (uint32_t j0 = wid * 64 * 8; j0 < nslots; j0 += 256 * 8) {
+                uint64_t kk[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t j = j0 + u * 64 + lane;
+                    kk[u] = j < nslots ? tab[j] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) fm.offer(kk[u], kk[u] != 0 && kk[u] >= floor_key);
+            }
+            fm.compact();
+            if (lane == 0) wave_count[wid][q] = fm.count;
+            __syncthreads();
+            if (wid == 0) {
+                for (uint32_t w = 1; w < 4; ++w) {
+                    const uint32_t c = wave_count[w][q];
+                    for (uint32_t i0 = 0; i0 < c; i0 += 64) {
+                        const uint32_t i = i0 + lane;
+                        fm.offer(i < c ? slots[w][q][i] : 0, i < c);
+                    }
+                }
+            }
+        } else if (wid == 0) {
+            fm.init(pool, POOL, a.k, nullptr, a.k_out);
+            fm.count = n;
+        }
+        if (wid == 0) {
+            fm.compact();
+            const uint64_t *list = fm.buf;
             uint32_t g = 0, e = 0;
             if ((uint32_t)q < a.nvalid)
-                heap_tie_shift(slots[0][q], tk[q].count, a.k_out,
+                heap_tie_shift(list, fm.count, a.k_out,
                                a.k + 1 >= 2 * a.k_out || a.k >= a.rows, g, e);
             for (uint32_t i = lane; i < a.k_out && (uint32_t)q < a.nvalid; i += 64) {
                 const uint32_t src_i = i < g ? i : i + e;
-                uint64_t key = src_i < tk[q].count ? slots[0][q][src_i] : 0;
-                a.out_idx[(size_t)q * a.out_stride + i] = key ? key_row(key) : 0;
-                a.out_sim[(size_t)q * a.out_stride + i] = key ? key_sim(key) : 0.f;
-            }
-            if (lane == 0) {
-                __hip_atomic_store(a.theta + q, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(a.part_count + q, 0u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t kv = src_i < fm.count ? list[src_i] : 0;
+                a.out_idx[(size_t)q * a.out_stride + i] = kv ? key_row(kv) : 0;
+                a.out_sim[(size_t)q * a.out_stride + i] = kv ? key_sim(kv) : 0.f;
             }
         }
-        if (lane == 0) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the pool and the slots are reused by the next query
+        __syncthreads();
     }
+    if (wid == 0 && lane == 0) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------

@@ -24,9 +24,9 @@ struct ScanArgs {
     uint32_t k;                   // keys selected per query (1 <= k <= rows)
     uint32_t k_out;               // neighbours written per query (<= k): when k >= 2 k_out - 1
                                   // the reference heap's choice among ties (heap_tie_shift)
-    uint64_t *part;               // [QB][grid * k] published keys, one dense list per query
-    uint32_t *part_count;         // [QB] keys published per query (0 between launches)
-    unsigned long long *theta;    // [QB] device-wide thresholds (0 between launches)
+    uint64_t *part;               // [QB][grid][k] each block's published list (key 0: empty)
+    uint32_t *part_count;         // [QB] not used by the scan (kept 0)
+    unsigned long long *theta;    // [QB] not used by the scan (kept 0)
     uint32_t *ticket;             // block-completion counter (0 between launches)
     uint64_t *out_idx;            // [QB][out_stride]
     float *out_sim;               // [QB][out_stride]
