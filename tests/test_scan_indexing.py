@@ -68,33 +68,41 @@ def replay(offsets, orig, probes, rows, k, large):
     assert grp_off[-1] <= gmax  # the launch covers every group
     worst_old = 0
     part_len = pairs * nchunk * kk
-    for g in range(int(grp_off[-1])):
+    # work items as the kernel decodes them (four waves per block, the grid
+    # rounded up): every (group, chunk) exactly once
+    seen = set()
+    for item in range(-(-(gmax * nchunk) // 4) * 4):
+        g, chunk = item // nchunk, item % nchunk
+        if g >= grp_off[-1]:
+            continue
         c = int(np.searchsorted(grp_off, g, side="right") - 1)
         assert grp_off[c] <= g < grp_off[c + 1]
+        assert (g, chunk) not in seen
+        seen.add((g, chunk))
         p0 = int(pair_off[c] + (g - grp_off[c]) * qb)
         pn = min(int(pair_off[c + 1]) - p0, qb)
         assert pn >= 1
         pids = order[p0:p0 + pn]
         l0, ln = int(offsets[c]), int(sizes[c])
         per = -(-ln // nchunk)
-        for chunk in range(nchunk):
-            r0 = l0 + min(chunk * per, ln)
-            r1 = l0 + min((chunk + 1) * per, ln)
-            row = np.arange(r0 // 64 * 64, r1, dtype=np.int64)
-            row = row[(row >= r0) & (row < r1)]
-            for pid in pids:
-                q = int(pid) // nprobe
-                abase = (int(base[pid]) - l0) % U32  # uint32 in the kernel
-                arr = (abase + row) % U32             # 32-bit sum: the arrival index
-                assert arr.size == 0 or arr.max() < totals[q]
-                if large:
-                    slot = q * ld + arr                 # shipped form
-                    assert slot.size == 0 or slot.max() < nq * ld
-                    if row.size:
-                        worst_old = max(worst_old, q * ld + abase + int(row.max()))
-                else:
-                    top = (int(pid) * nchunk + chunk) * kk + kk - 1
-                    assert top < part_len
+        r0 = l0 + min(chunk * per, ln)
+        r1 = l0 + min((chunk + 1) * per, ln)
+        row = np.arange(r0 // 64 * 64, r1, dtype=np.int64)
+        row = row[(row >= r0) & (row < r1)]
+        for pid in pids:
+            q = int(pid) // nprobe
+            abase = (int(base[pid]) - l0) % U32  # uint32 in the kernel
+            arr = (abase + row) % U32             # 32-bit sum: the arrival index
+            assert arr.size == 0 or arr.max() < totals[q]
+            if large:
+                slot = q * ld + arr                 # shipped form
+                assert slot.size == 0 or slot.max() < nq * ld
+                if row.size:
+                    worst_old = max(worst_old, q * ld + abase + int(row.max()))
+            else:
+                top = (int(pid) * nchunk + chunk) * kk + kk - 1
+                assert top < part_len
+    assert len(seen) == int(grp_off[-1]) * nchunk
     # the merge / finisher lookup: arrival -> probe -> reordered -> original
     for q in range(nq):
         pb = base[q * nprobe:(q + 1) * nprobe]
