@@ -5,8 +5,7 @@ of the merge form and the merge's arrival -> original row lookup, each
 checked against the length of the buffer it addresses.
 
 The data are the oracle's own IVF build and probes (orc.Ivf) at the shapes
-the GPU tests run, including the case the round's GPU memory fault came from
-(large k: the key slot was formed as size_t + uint32 + uint32, so a 32-bit
+the GPU tests run, including the large-k case that faulted the GPU (the key slot was formed as size_t + uint32 + uint32, so a 32-bit
 arrival base that wraps below zero was widened before it could wrap back,
 landing 2^32 keys past the array); the pre-fix form is replayed beside the
 shipped one and must leave the array.  The arithmetic restates
@@ -125,7 +124,7 @@ def _ivf(orc, rows, dims, seed):
 
 
 @pytest.mark.parametrize("rows,dims,k,nprobe,nq", [
-    (8000, 64, 20, 3, 12),     # the round-one GPU fault's case (merge form)
+    (8000, 64, 20, 3, 12),     # the merge-form case kept off the GPU (unexplained fault)
     (8000, 64, 20, 3, 1),
     (30000, 32, 481, 20, 7),   # the large-k form whose key slot faulted
     (30000, 32, 3000, 24, 1),
