@@ -158,6 +158,34 @@ def test_ties_follow_the_reference_heap(orc):
     assert shifted > 0  # the case exercises the heap's own choice
 
 
+def test_final_merge_with_every_block_maximum_tied(orc):
+    """200,000 rows drawn from 8 vectors: every scan block's best keys tie at
+    the top score, so every published key reaches the final merge's bound.
+    At k = 10 the 196 blocks x 19 kept keys overflow the merge's LDS pool
+    (512 keys for one query, 2,048 for the 4-query form of a 3-query batch)
+    and take the fallback in which every wave offers its share of the table;
+    at k = 1 the 196 keys fit the pool.  Single queries and the batch must
+    equal the oracle's restatement of the reference GPU path (its heap keeps
+    the lowest tied rows here)."""
+    _print_case("final_merge_ties", rows=200000, dims=32)
+    rng = np.random.default_rng(21)
+    pool = rng.normal(size=(8, 32)).astype(np.float32)
+    m = np.ascontiguousarray(pool[rng.integers(0, 8, size=200000)])
+    qs = np.ascontiguousarray(pool[[3, 5, 0]] + 0.05 * rng.normal(size=(3, 32)).astype(np.float32))
+    om = orc.normalize_rows(m, m.shape[0], 32)
+    oq = orc.normalize_rows(qs, 3, 32)
+    index = nndex_amd.NNdex(m, normalized=False, backend="gpu", enable_cache=False)
+    for k in (10, 1):
+        bidx, bsim = index.search(qs, k=k)
+        for i in range(3):
+            sidx, ssim = index.search(qs[i], k=k)
+            o1idx, o1sim = orc.gpu_exact_search(om, orc.normalize_vector(qs[i]), k)
+            assert sidx.tolist() == o1idx.tolist() and ssim.tolist() == o1sim.tolist(), (k, i)
+            oidx, osim = orc.gpu_exact_search(om, oq[i], k)
+            assert bidx[i].tolist() == oidx.tolist() and bsim[i].tolist() == osim.tolist(), (k, i)
+            assert len(set(oidx.tolist())) == k and len(set(osim.tolist())) == 1
+
+
 def test_all_negative_scores_and_k_capping():
     """src/lib.rs:1466-1506 and :1374-1425."""
     _print_case("negatives")
