@@ -64,6 +64,19 @@ def test_c2_single_query_kernel(orc, c2):
             full = orc.fp64_scores(base, qs[i], normalized=True)
             fidx, _ = orc.fp64_topk(base, qs[i], 10, normalized=True)
             assert_topk_parity(gi, gs, fidx[0], full)
+    # the benchmark's launch shape (20 tiles per wave: 196 blocks, whose
+    # final merge finds its bound by ballots over the block maxima; the
+    # default shape above has more than 512 blocks and skips that step)
+    from nndex_amd import _lib
+    _lib.check(_lib.lib().nndex_gpu_index_set_launch(b.h, 20, 1))
+    ds20 = DeviceSearch(b.h, 1, 10)
+    for i in range(8):
+        dq = DevBuf.from_array(qs[i:i + 1])
+        ds20.run(dq, 1, 10, di, dsim)
+        gi, gs = di.to_array((10,), np.uint64), dsim.to_array((10,), np.float32)
+        oidx, osim, _ = orc.cpu_search(base, qs[i], 10)
+        assert gi.tolist() == oidx.tolist(), i
+        assert np.array_equal(gs.view(np.uint32), osim.view(np.uint32)), i
 
 
 def test_c2_batch_4096_through_frontend(orc, c2):
