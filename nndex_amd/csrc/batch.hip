@@ -1140,6 +1140,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
                 const float4 *xb = a.x + (size_t)row * a.d4;
                 const float4 *qb = a.q + (size_t)q * a.d4;
                 float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+                // unrolled so a 128-byte run's loads are issued together
+#pragma unroll 8
                 for (uint32_t cc = 0; cc < a.d4; ++cc) {
                     float4 xv = xb[cc], qv = qb[cc];
                     l0 = __builtin_fmaf(xv.x, qv.x, l0);
@@ -1209,6 +1211,7 @@ __global__ __launch_bounds__(256) void rerank_full_kernel(RerankArgs a) {
             const uint32_t row = key_row(c[i]);
             const float4 *xb = a.x + (size_t)row * a.d4;
             float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+#pragma unroll 8
             for (uint32_t cc = 0; cc < a.d4; ++cc) {
                 float4 xv = xb[cc], qv = qb[cc];
                 l0 = __builtin_fmaf(xv.x, qv.x, l0);
