@@ -1087,8 +1087,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
         // through LDS in segments of kSeg float4 (kSeg lanes read one row's
         // 128-byte run, rather than 64 lanes each reading its own row),
         // then every lane runs its row's four FMA chains over the
-        // segment.  The next segment's loads are in flight while the
-        // current one is summed.
+        // segment.  The loads of the next two segments are in flight
+        // while one is summed.
         constexpr uint32_t kSeg = 8, kPad = kSeg + 1;  // padded rows: no bank conflicts
         float4 *stg = stage[CW >= 512 ? wid : 0];
         const float4 *qrow = a.q + (size_t)q * a.d4;
@@ -1104,21 +1104,21 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
                     const uint32_t ci = wbase + 8 * j + lane / kSeg;
                     crow[j] = ci < nres ? key_row(slots[0][ci]) : 0xFFFFFFFFu;
                 }
-                float4 v[8];
-                auto load = [&](uint32_t c0) {
+                // two register sets: segment s + 2 is loading while s is summed
+                float4 va[8], vb[8];
+                auto load = [&](float4(&v)[8], uint32_t c0) {
                     const uint32_t cc = c0 + lane % kSeg;
     #pragma unroll
                     for (int j = 0; j < 8; ++j)
                         v[j] = crow[j] != 0xFFFFFFFFu && cc < a.d4 ? a.x[(size_t)crow[j] * a.d4 + cc]
                                                                    : make_float4(0.f, 0.f, 0.f, 0.f);
                 };
-                load(0);
-                for (uint32_t c0 = 0; c0 < a.d4; c0 += kSeg) {
+                auto segment = [&](float4(&v)[8], uint32_t c0) {
                     wave_lds_sync();  // the previous segment is summed
     #pragma unroll
                     for (int j = 0; j < 8; ++j) stg[(8 * j + lane / kSeg) * kPad + lane % kSeg] = v[j];
                     wave_lds_sync();
-                    if (c0 + kSeg < a.d4) load(c0 + kSeg);
+                    if (c0 + 2 * kSeg < a.d4) load(v, c0 + 2 * kSeg);
                     const uint32_t nc = a.d4 - c0 < kSeg ? a.d4 - c0 : kSeg;
                     for (uint32_t cc = 0; cc < nc; ++cc) {
                         const float4 xv = stg[lane * kPad + cc], qv = qrow[c0 + cc];
@@ -1127,6 +1127,12 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
                         l2 = __builtin_fmaf(xv.z, qv.z, l2);
                         l3 = __builtin_fmaf(xv.w, qv.w, l3);
                     }
+                };
+                load(va, 0);
+                if (kSeg < a.d4) load(vb, kSeg);
+                for (uint32_t c0 = 0; c0 < a.d4; c0 += 2 * kSeg) {
+                    segment(va, c0);
+                    if (c0 + kSeg < a.d4) segment(vb, c0 + kSeg);
                 }
             }
             if (i < CW) exact[i] = i < nres ? make_key((l0 + l1) + (l2 + l3), key_row(slots[0][i])) : 0;
