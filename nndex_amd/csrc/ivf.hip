@@ -52,6 +52,8 @@ __device__ inline float tiled_dot(const float4 *__restrict__ x, uint32_t row, ui
                                   const float4 *__restrict__ v) {
     const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
     float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+    // unrolled: the row's loads are requested together, not one per step
+#pragma unroll 8
     for (uint32_t c = 0; c < d4; ++c) {
         const float4 a = b[(size_t)c * 64], q = v[c];
         l0 = __builtin_fmaf(a.x, q.x, l0);
@@ -216,6 +218,7 @@ __global__ __launch_bounds__(kKppRows) void kpp_update_kernel(const float4 *__re
             const uint4 *b = xb + (size_t)(row / 64) * d8 * 64 + row % 64;
             const_as<f32x4> *cb = as_constant(reinterpret_cast<const f32x4 *>(lastb));
             float l[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
             for (uint32_t c = 0; c < d8; ++c) {
                 const uint4 w = b[(size_t)c * 64];
                 const f32x4 q0 = cb[2 * c], q1 = cb[2 * c + 1];
@@ -267,6 +270,7 @@ __global__ __launch_bounds__(kKppRows) void kpp_update_fp8_kernel(
             const uint4 *b = x8 + (size_t)(row / 64) * d16 * 64 + row % 64;
             const_as<f32x4> *cb = as_constant(reinterpret_cast<const f32x4 *>(last));
             float l[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
             for (uint32_t u = 0; u < d16; ++u) {
                 const uint4 w = b[(size_t)u * 64];
                 const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
@@ -522,6 +526,7 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float4 *__restrict__
         for (uint32_t cl = threadIdx.x; cl < nlist; cl += 256) {
             const float4 *cv = centroids + (size_t)cl * d4;
             float l0 = 0.f, l1 = 0.f, l2 = 0.f, l3 = 0.f;
+#pragma unroll 8
             for (uint32_t c = 0; c < d4; ++c) {
                 const float4 a = srow[c], b = cv[c];
                 l0 = __builtin_fmaf(a.x, b.x, l0);
