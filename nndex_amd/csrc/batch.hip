@@ -252,6 +252,7 @@ __global__ __launch_bounds__(256) void to_fp8_kernel(const float4 *__restrict__ 
     if (row < rows_padded) {
         const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
         float mx = 0.f;
+#pragma unroll 8
         for (uint32_t c = 0; c < d4; ++c) {
             const float4 v = b[(size_t)c * 64];
             mx = __builtin_fmaxf(mx, __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(v.x),
@@ -324,6 +325,7 @@ __global__ __launch_bounds__(256) void bf16_error_kernel(const float4 *__restric
     double ss = 0.0;
     if (row < rows) {
         const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
+#pragma unroll 8
         for (uint32_t c = 0; c < d4; ++c) {
             float4 v = b[(size_t)c * 64];
             float e[4] = {v.x, v.y, v.z, v.w};
@@ -349,6 +351,7 @@ __global__ __launch_bounds__(256) void max_norm_kernel(const float4 *__restrict_
     double ss = 0.0;
     if (row < rows) {
         const float4 *b = x + (size_t)(row / 64) * d4 * 64 + row % 64;
+#pragma unroll 8
         for (uint32_t c = 0; c < d4; ++c) {
             float4 v = b[(size_t)c * 64];
             ss += (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w;
